@@ -1,0 +1,168 @@
+/* ket_amd — C ABI of the MI355X (gfx950) state-vector backend for ket.
+ *
+ * Plain C types only: an opaque device state handle, 64-bit sizes and masks,
+ * host pointers to doubles. Every call returns a ket_status; on failure
+ * ket_last_error() holds a message for the calling thread. The C++ layer
+ * (the ket:: headers under include/ket) maps statuses back to ket's exception types
+ * (std::out_of_range, std::invalid_argument, std::runtime_error).
+ *
+ * Each entry point names the reference function whose work it takes over
+ * (file:line into the ket source tree). The reference has no FFI of its own:
+ * its seam is the C++ engine below ket::run / ket::simulate, and these calls
+ * are what that engine is rebuilt on.
+ *
+ * Amplitude layout in HBM: 2^n interleaved complex numbers, index bit q =
+ * qubit q (little-endian, reference README.md:177-179); complex128 (16 B) or
+ * complex64 (8 B) per amplitude, one contiguous device allocation.
+ */
+#ifndef KET_AMD_H
+#define KET_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ket_status {
+  KET_OK = 0,
+  KET_ERR_ARG = 1,         /* invalid argument (std::invalid_argument) */
+  KET_ERR_RANGE = 2,       /* qubit index out of range (std::out_of_range) */
+  KET_ERR_DEVICE = 3,      /* HIP runtime error */
+  KET_ERR_ALLOC = 4,       /* device allocation failed */
+  KET_ERR_UNSUPPORTED = 5  /* valid request this build does not handle */
+} ket_status;
+
+typedef enum ket_precision {
+  KET_C128 = 0, /* complex128: ket's own precision */
+  KET_C64 = 1   /* complex64 amplitudes; norms and probabilities in FP64 */
+} ket_precision;
+
+/* Gate type codes: the order of ket::GateType (include/ket/gate.hpp:16-29). */
+typedef enum ket_gate_type {
+  KET_H, KET_X, KET_Y, KET_Z, KET_S, KET_SDG, KET_T, KET_TDG,
+  KET_RX, KET_RY, KET_RZ, KET_U,
+  KET_CH, KET_CX, KET_CY, KET_CZ, KET_CRX, KET_CRY, KET_CRZ, KET_CU, KET_CP,
+  KET_SWAP, KET_CCX, KET_CSWAP,
+  KET_MEASURE, KET_BARRIER, KET_COMPOSITE, KET_PROBE
+} ket_gate_type;
+
+/* One flattened gate (48 bytes). qubits[] are state qubits in the gate's own
+ * operand order (controls first, as ket::Circuit records them). Structural
+ * records (MEASURE, BARRIER, PROBE) carry no work and end a fusion block, as
+ * ket's driver flushes its Fuser on them (src/simulator.cpp:496). A composite
+ * is flattened by the caller into BARRIER, its gates, BARRIER. */
+typedef struct ket_gate {
+  int32_t type;
+  int32_t n_qubits;
+  int32_t qubits[3];
+  int32_t pad;
+  double params[3];
+} ket_gate;
+
+/* How a circuit becomes kernel launches. */
+typedef enum ket_fusion {
+  KET_FUSE_NONE = 0,      /* one launch per gate (ket's Stepper/apply_node) */
+  KET_FUSE_REFERENCE = 1, /* ket's greedy same-qubit-set Fuser
+                             (src/simulator.cpp:400-477); with KET_C128 the
+                             result equals ket::run's value for value */
+  KET_FUSE_BLOCKS = 2,    /* k-qubit packer: dense 2^k blocks (k = block_k) */
+  KET_FUSE_LEARNED = 3    /* packer whose block boundaries a trained cost
+                             model picks; falls back to KET_FUSE_BLOCKS */
+} ket_fusion;
+
+typedef struct ket_sv ket_sv; /* a state vector resident on one GPU */
+
+/* Plan and launch statistics of the last ket_sv_apply_circuit call. */
+typedef struct ket_run_stats {
+  uint64_t gates;      /* unitary gates in the input */
+  uint64_t launches;   /* state passes launched */
+  uint64_t blocks;     /* fused dense blocks among them */
+  double alg_bytes;    /* algorithmic HBM bytes of the launches (line model) */
+  double flops;        /* floating-point operations of the launches */
+  double plan_seconds; /* host planning time */
+} ket_run_stats;
+
+const char* ket_last_error(void);
+
+/* Device name, compute units and HBM bytes of the current device. */
+int ket_device_info(char* name, size_t name_len, int* compute_units,
+                    uint64_t* hbm_bytes);
+
+/* ---- state lifetime: replaces ground_state / State storage
+ *      (src/simulator.cpp:529-533, include/ket/state.hpp:15-34) ---- */
+int ket_sv_create(int n_qubits, int precision, ket_sv** out);
+int ket_sv_destroy(ket_sv* sv);
+int ket_sv_info(const ket_sv* sv, int* n_qubits, int* precision);
+int ket_sv_set_basis(ket_sv* sv, uint64_t index); /* |index>; 0 = |0...0> */
+/* Interleaved (re, im) doubles; count amplitudes from offset. */
+int ket_sv_copy_in(ket_sv* sv, uint64_t offset, uint64_t count,
+                   const double* src);
+int ket_sv_copy_out(ket_sv* sv, uint64_t offset, uint64_t count, double* dst);
+/* Device-to-device copy (probe snapshots, src/simulator.cpp:501-503); both
+ * states must have the same size and precision. */
+int ket_sv_copy(ket_sv* dst, ket_sv* src);
+int ket_sv_sync(ket_sv* sv);
+
+/* ---- per-gate kernels: the plain form (src/simulator.cpp:23-254) ---- */
+/* Dense 2x2 (row-major, interleaved complex: m00 m01 m10 m11) on target,
+ * applied where every bit of ctrl_mask is 1. apply_single (:23-36) and
+ * apply_controlled_single (:148-164). */
+int ket_sv_apply_1q(ket_sv* sv, int target, uint64_t ctrl_mask,
+                    const double m[8]);
+/* Multiply by (re, im) where every bit of mask is 1: apply_cphase
+ * (:246-254) and the diagonal single-qubit gates. */
+int ket_sv_apply_phase(ket_sv* sv, uint64_t mask, double re, double im);
+/* Exchange the target pairs where every bit of ctrl_mask is 1: X, CX
+ * (:99-109), CCX (:215-228). */
+int ket_sv_apply_x(ket_sv* sv, int target, uint64_t ctrl_mask);
+/* Exchange |..a=1..b=0..> with |..a=0..b=1..> where ctrl_mask is 1: SWAP
+ * (:204-212), CSWAP (:232-242). */
+int ket_sv_apply_swap(ket_sv* sv, int a, int b, uint64_t ctrl_mask);
+/* One gate record through ket's gate dispatch (apply_gate_on, :259-338). */
+int ket_sv_apply_gate(ket_sv* sv, const ket_gate* gate);
+/* A 2^k x 2^k row-major complex matrix on qubits[0..k), bit b of a matrix
+ * index = qubits[b] (apply_matrix, :357-393). k <= 6. */
+int ket_sv_apply_matrix(ket_sv* sv, int k, const int* qubits,
+                        const double* mat);
+
+/* ---- circuits: apply_circuit + Fuser (src/simulator.cpp:400-517) ---- */
+/* block_k: largest fused block for KET_FUSE_BLOCKS/LEARNED (2..6). stats may
+ * be NULL. */
+int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
+                         int fusion, int block_k, ket_run_stats* stats);
+
+/* ---- reductions (src/measurement.cpp:12-21, src/observable.cpp:18-91) ---- */
+int ket_sv_norm(ket_sv* sv, double* out); /* sum |a|^2 in FP64 */
+/* Scale so that the FP64 norm is 1 (the post-measurement renormalisation of
+ * a state kept in complex64). */
+int ket_sv_renormalize(ket_sv* sv);
+/* Born-rule outcomes for n uniforms u[i] in [0,1). Each outcome equals what
+ * ket's measure() returns for that u: the first index whose sequential FP64
+ * cumulative sum of |a|^2 exceeds u, else 2^n - 1. */
+int ket_sv_sample(ket_sv* sv, const double* u, uint64_t n, uint64_t* out);
+/* <psi|P|psi> for the masks ket's expval derives from a Pauli string:
+ * flip = X|Y positions, sign = Z|Y positions, n_y = number of Y. */
+int ket_sv_expval_pauli(ket_sv* sv, uint64_t flip, uint64_t sign, int n_y,
+                        double* out);
+/* sum_t coeff[t] <P_t> for n_terms Pauli terms in one call; diagonal terms
+ * (flip = 0) share one pass over the state. */
+int ket_sv_expval_pauli_sum(ket_sv* sv, size_t n_terms, const uint64_t* flip,
+                            const uint64_t* sign, const int* n_y,
+                            const double* coeff, double* out);
+/* F = |<a|b>|^2 / (<a|a><b|b>) in FP64 and max_i |a_i - b_i| over two states
+ * of one size (any precisions). */
+int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity,
+                    double* max_abs_diff);
+
+/* ---- timing on the state's stream ---- */
+/* Record event slot (0..7) on the state's stream; elapsed ms between two. */
+int ket_sv_event_record(ket_sv* sv, int slot);
+int ket_sv_event_elapsed(ket_sv* sv, int from, int to, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KET_AMD_H */

@@ -1,0 +1,635 @@
+// C ABI of the gfx950 state-vector backend (declarations: include/ket_amd.h).
+//
+// A ket_sv owns one device allocation of 2^n amplitudes, one HIP stream (at
+// the default priority) and small device scratch for reductions, matrices
+// and the sampler's chunk tables. Every call validates its arguments on the
+// host before any launch, so a bad qubit index or size becomes a status code,
+// never an out-of-bounds kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "ket_amd.h"
+#include "plan.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? KET_ERR_ALLOC : KET_ERR_DEVICE,
+              std::string("ket_amd: ") + what + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                     \
+  do {                                          \
+    hipError_t e_ = (expr);                     \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+constexpr int kMaxQubits = 40;
+
+}  // namespace
+
+struct ket_sv {
+  int n = 0;
+  int precision = 0;
+  uint64_t dim = 0;
+  void* data = nullptr;
+  hipStream_t stream = nullptr;
+  double* work = nullptr;    // reduction partials
+  double* result = nullptr;  // small device results (8 doubles)
+  void* pool = nullptr;      // matrices / tables of the current plan
+  size_t pool_cap = 0;
+  void* pool_host = nullptr; // pinned staging for the pool upload
+  size_t pool_host_cap = 0;
+  hipEvent_t pool_ev = nullptr;  // recorded after the last pool upload
+  void* sbuf = nullptr;      // sampler chunk tables
+  bool sample_valid = false;
+  void* io = nullptr;        // shot uniforms / outcomes / pauli terms
+  size_t io_cap = 0;
+  hipEvent_t ev[8] = {};
+  ket_run_stats last{};
+};
+
+namespace {
+
+size_t amp_bytes(int precision) { return precision == 0 ? 16 : 8; }
+
+int ensure(void** p, size_t* cap, size_t bytes, const char* what) {
+  if (*cap >= bytes) return KET_OK;
+  const size_t want = std::max(bytes, *cap * 2);
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc(p, want), what);
+  *cap = want;
+  return KET_OK;
+}
+
+int check_sv(const ket_sv* sv) {
+  if (!sv || !sv->data) return fail(KET_ERR_ARG, "ket_amd: null state");
+  return KET_OK;
+}
+
+int check_qubit(const ket_sv* sv, int q) {
+  if (q < 0 || q >= sv->n)
+    return fail(KET_ERR_RANGE, "ket: qubit index " + std::to_string(q) +
+                                   " is out of range for a " +
+                                   std::to_string(sv->n) + "-qubit circuit");
+  return KET_OK;
+}
+
+int check_mask(const ket_sv* sv, uint64_t mask) {
+  if (sv->n < 64 && (mask >> sv->n) != 0)
+    return fail(KET_ERR_RANGE, "ket_amd: mask has bits beyond the state's qubits");
+  return KET_OK;
+}
+
+std::vector<int> mask_bits(uint64_t m) {
+  std::vector<int> v;
+  for (int q = 0; q < 64; ++q)
+    if (m & (uint64_t{1} << q)) v.push_back(q);
+  return v;
+}
+
+ketk::PairSpec spec_for(int n, std::vector<int> fixed, uint64_t set,
+                        uint64_t partner) {
+  std::sort(fixed.begin(), fixed.end());
+  ketk::PairSpec sp{};
+  sp.npos = (int)fixed.size();
+  for (int i = 0; i < sp.npos; ++i) sp.pos[i] = fixed[i];
+  sp.n_items = (uint64_t{1} << n) >> sp.npos;
+  sp.set_mask = set;
+  sp.partner = partner;
+  return sp;
+}
+
+// Launch one planned pass; pool_dev is the device copy of plan.pool.
+int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
+  hipError_t e = hipSuccess;
+  switch (o.kind) {
+    case ketplan::Kind::Pair:
+      e = ketk::launch_pair(sv->stream, sv->precision, sv->data, o.spec,
+                            o.pair_op, o.m);
+      break;
+    case ketplan::Kind::Phase:
+      e = ketk::launch_phase(sv->stream, sv->precision, sv->data, o.spec,
+                             o.m[0], o.m[1]);
+      break;
+    case ketplan::Kind::MatrixExact:
+      e = ketk::launch_matrix_exact(sv->stream, sv->precision, sv->data, sv->n,
+                                    o.k, o.qubits, pool_dev + o.pool_offset);
+      break;
+    case ketplan::Kind::Block:
+      e = ketk::launch_block(sv->stream, sv->precision, sv->data, sv->n, o.k,
+                             o.qubits, pool_dev + o.pool_offset);
+      break;
+  }
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return KET_OK;
+}
+
+int run_plan(ket_sv* sv, const ketplan::Plan& plan) {
+  const unsigned char* pool_dev = nullptr;
+  if (!plan.pool.empty()) {
+    // The previous upload must have left the pinned staging buffer before it
+    // is rewritten; kernels of earlier plans that still read the device pool
+    // are ordered before this copy on the same stream.
+    HIP_TRY(hipEventSynchronize(sv->pool_ev), "hipEventSynchronize(pool)");
+    int st = ensure(&sv->pool, &sv->pool_cap, plan.pool.size(), "hipMalloc(pool)");
+    if (st) return st;
+    if (sv->pool_host_cap < plan.pool.size()) {
+      if (sv->pool_host) (void)hipHostFree(sv->pool_host);
+      sv->pool_host = nullptr;
+      sv->pool_host_cap = 0;
+      const size_t want = std::max(plan.pool.size(), size_t{1} << 20);
+      HIP_TRY(hipHostMalloc(&sv->pool_host, want, hipHostMallocDefault), "hipHostMalloc(pool)");
+      sv->pool_host_cap = want;
+    }
+    std::memcpy(sv->pool_host, plan.pool.data(), plan.pool.size());
+    HIP_TRY(hipMemcpyAsync(sv->pool, sv->pool_host, plan.pool.size(),
+                           hipMemcpyHostToDevice, sv->stream),
+            "hipMemcpyAsync(pool)");
+    HIP_TRY(hipEventRecord(sv->pool_ev, sv->stream), "hipEventRecord(pool)");
+    pool_dev = static_cast<const unsigned char*>(sv->pool);
+  }
+  for (const ketplan::Op& o : plan.ops) {
+    int st = launch_op(sv, o, pool_dev);
+    if (st) return st;
+  }
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+void fill_stats(const ket_sv* sv, const ketplan::Plan& plan, double plan_s,
+                ket_run_stats* st) {
+  ket_run_stats r{};
+  r.gates = plan.gates;
+  r.launches = plan.ops.size();
+  r.blocks = plan.blocks;
+  const double sweep = 2.0 * (double)amp_bytes(sv->precision) * (double)sv->dim;
+  for (const ketplan::Op& o : plan.ops) {
+    r.alg_bytes += sweep * o.line_fraction;
+    double touched = (double)sv->dim;
+    if (o.kind == ketplan::Kind::Phase) touched = (double)o.spec.n_items;
+    else if (o.kind == ketplan::Kind::Pair) touched = 2.0 * (double)o.spec.n_items;
+    r.flops += o.flops_per_amp * touched;
+  }
+  r.plan_seconds = plan_s;
+  if (st) *st = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ket_last_error(void) { return g_err.c_str(); }
+
+int ket_device_info(char* name, size_t name_len, int* compute_units,
+                    uint64_t* hbm_bytes) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (compute_units) *compute_units = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (uint64_t)p.totalGlobalMem;
+  return KET_OK;
+}
+
+int ket_sv_create(int n_qubits, int precision, ket_sv** out) {
+  if (!out) return fail(KET_ERR_ARG, "ket_amd: null output handle");
+  *out = nullptr;
+  if (n_qubits < 0 || n_qubits > kMaxQubits)
+    return fail(KET_ERR_ARG, "ket_amd: qubit count " + std::to_string(n_qubits) +
+                                 " outside [0, 40]");
+  if (precision != KET_C128 && precision != KET_C64)
+    return fail(KET_ERR_ARG, "ket_amd: unknown precision");
+  auto* sv = new ket_sv;
+  sv->n = n_qubits;
+  sv->precision = precision;
+  sv->dim = uint64_t{1} << n_qubits;
+  const size_t bytes = (size_t)sv->dim * amp_bytes(precision);
+  hipError_t e = hipMalloc(&sv->data, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    delete sv;
+    return fail(KET_ERR_ALLOC, "ket_amd: cannot allocate " + std::to_string(bytes) +
+                                   " bytes of device memory for a " +
+                                   std::to_string(n_qubits) + "-qubit state: " +
+                                   hipGetErrorString(e));
+  }
+  auto cleanup = [&](int code) {
+    ket_sv_destroy(sv);
+    return code;
+  };
+  if ((e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking)) != hipSuccess)
+    return cleanup(hip_fail(e, "hipStreamCreate"));
+  if ((e = hipMalloc(&sv->work, ketk::reduce_work_bytes(sv->dim))) != hipSuccess)
+    return cleanup(hip_fail(e, "hipMalloc(work)"));
+  if ((e = hipMalloc(&sv->result, 16 * sizeof(double))) != hipSuccess)
+    return cleanup(hip_fail(e, "hipMalloc(result)"));
+  for (auto& ev : sv->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess)
+      return cleanup(hip_fail(e, "hipEventCreate"));
+  if ((e = hipEventCreateWithFlags(&sv->pool_ev, hipEventDisableTiming)) != hipSuccess)
+    return cleanup(hip_fail(e, "hipEventCreate"));
+  if ((e = hipEventRecord(sv->pool_ev, sv->stream)) != hipSuccess)
+    return cleanup(hip_fail(e, "hipEventRecord"));
+  if ((e = ketk::launch_fill_basis(sv->stream, precision, sv->data, sv->dim, 0)) !=
+      hipSuccess)
+    return cleanup(hip_fail(e, "fill"));
+  *out = sv;
+  return KET_OK;
+}
+
+int ket_sv_destroy(ket_sv* sv) {
+  if (!sv) return KET_OK;
+  if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+  for (auto& ev : sv->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (sv->data) (void)hipFree(sv->data);
+  if (sv->work) (void)hipFree(sv->work);
+  if (sv->result) (void)hipFree(sv->result);
+  if (sv->pool) (void)hipFree(sv->pool);
+  if (sv->pool_host) (void)hipHostFree(sv->pool_host);
+  if (sv->pool_ev) (void)hipEventDestroy(sv->pool_ev);
+  if (sv->sbuf) (void)hipFree(sv->sbuf);
+  if (sv->io) (void)hipFree(sv->io);
+  if (sv->stream) (void)hipStreamDestroy(sv->stream);
+  delete sv;
+  return KET_OK;
+}
+
+int ket_sv_info(const ket_sv* sv, int* n_qubits, int* precision) {
+  if (int st = check_sv(sv)) return st;
+  if (n_qubits) *n_qubits = sv->n;
+  if (precision) *precision = sv->precision;
+  return KET_OK;
+}
+
+int ket_sv_set_basis(ket_sv* sv, uint64_t index) {
+  if (int st = check_sv(sv)) return st;
+  if (index >= sv->dim) return fail(KET_ERR_RANGE, "ket_amd: basis index out of range");
+  HIP_TRY(ketk::launch_fill_basis(sv->stream, sv->precision, sv->data, sv->dim, index),
+          "fill");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_copy_in(ket_sv* sv, uint64_t offset, uint64_t count, const double* src) {
+  if (int st = check_sv(sv)) return st;
+  if (offset > sv->dim || count > sv->dim - offset)
+    return fail(KET_ERR_RANGE, "ket_amd: copy range outside the state");
+  if (count == 0) return KET_OK;
+  if (!src) return fail(KET_ERR_ARG, "ket_amd: null source");
+  if (sv->precision == KET_C128) {
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(sv->data) + offset * 16, src, count * 16,
+                           hipMemcpyHostToDevice, sv->stream),
+            "hipMemcpyAsync");
+  } else {
+    // Stage complex128 on the device, narrow there.
+    const uint64_t step = uint64_t{1} << 24;
+    int st = ensure(&sv->io, &sv->io_cap, (size_t)std::min(count, step) * 16, "hipMalloc(io)");
+    if (st) return st;
+    for (uint64_t o = 0; o < count; o += step) {
+      const uint64_t c = std::min(step, count - o);
+      HIP_TRY(hipMemcpyAsync(sv->io, src + 2 * o, c * 16, hipMemcpyHostToDevice, sv->stream),
+              "hipMemcpyAsync");
+      HIP_TRY(ketk::launch_convert(sv->stream, 1,
+                                   static_cast<char*>(sv->data) + (offset + o) * 8, 0,
+                                   sv->io, c),
+              "convert");
+      HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_copy_out(ket_sv* sv, uint64_t offset, uint64_t count, double* dst) {
+  if (int st = check_sv(sv)) return st;
+  if (offset > sv->dim || count > sv->dim - offset)
+    return fail(KET_ERR_RANGE, "ket_amd: copy range outside the state");
+  if (count == 0) return KET_OK;
+  if (!dst) return fail(KET_ERR_ARG, "ket_amd: null destination");
+  if (sv->precision == KET_C128) {
+    HIP_TRY(hipMemcpyAsync(dst, static_cast<char*>(sv->data) + offset * 16, count * 16,
+                           hipMemcpyDeviceToHost, sv->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+    return KET_OK;
+  }
+  const uint64_t step = uint64_t{1} << 24;
+  int st = ensure(&sv->io, &sv->io_cap, (size_t)std::min(count, step) * 16, "hipMalloc(io)");
+  if (st) return st;
+  for (uint64_t o = 0; o < count; o += step) {
+    const uint64_t c = std::min(step, count - o);
+    HIP_TRY(ketk::launch_convert(sv->stream, 0, sv->io, 1,
+                                 static_cast<char*>(sv->data) + (offset + o) * 8, c),
+            "convert");
+    HIP_TRY(hipMemcpyAsync(dst + 2 * o, sv->io, c * 16, hipMemcpyDeviceToHost, sv->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  }
+  return KET_OK;
+}
+
+int ket_sv_copy(ket_sv* dst, ket_sv* src) {
+  if (int st = check_sv(dst)) return st;
+  if (int st = check_sv(src)) return st;
+  if (dst->n != src->n) return fail(KET_ERR_ARG, "ket_amd: copy between different sizes");
+  HIP_TRY(hipStreamSynchronize(src->stream), "sync");
+  if (dst->precision == src->precision) {
+    HIP_TRY(hipMemcpyAsync(dst->data, src->data, (size_t)dst->dim * amp_bytes(dst->precision),
+                           hipMemcpyDeviceToDevice, dst->stream),
+            "hipMemcpyAsync");
+  } else {
+    HIP_TRY(ketk::launch_convert(dst->stream, dst->precision, dst->data, src->precision,
+                                 src->data, dst->dim),
+            "convert");
+  }
+  HIP_TRY(hipStreamSynchronize(dst->stream), "sync");
+  dst->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_sync(ket_sv* sv) {
+  if (int st = check_sv(sv)) return st;
+  HIP_TRY(hipStreamSynchronize(sv->stream), "hipStreamSynchronize");
+  return KET_OK;
+}
+
+int ket_sv_apply_1q(ket_sv* sv, int target, uint64_t ctrl_mask, const double m[8]) {
+  if (int st = check_sv(sv)) return st;
+  if (int st = check_qubit(sv, target)) return st;
+  if (int st = check_mask(sv, ctrl_mask)) return st;
+  if (!m) return fail(KET_ERR_ARG, "ket_amd: null matrix");
+  if (ctrl_mask & (uint64_t{1} << target))
+    return fail(KET_ERR_ARG, "ket_amd: target is also a control");
+  std::vector<int> fixed = mask_bits(ctrl_mask);
+  if (fixed.size() > 3) return fail(KET_ERR_UNSUPPORTED, "ket_amd: at most 3 controls");
+  fixed.push_back(target);
+  const ketk::PairSpec sp = spec_for(sv->n, fixed, ctrl_mask, uint64_t{1} << target);
+  HIP_TRY(ketk::launch_pair(sv->stream, sv->precision, sv->data, sp, ketk::OP_DENSE, m),
+          "apply_1q");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_apply_phase(ket_sv* sv, uint64_t mask, double re, double im) {
+  if (int st = check_sv(sv)) return st;
+  if (int st = check_mask(sv, mask)) return st;
+  std::vector<int> fixed = mask_bits(mask);
+  if (fixed.size() > 4) return fail(KET_ERR_UNSUPPORTED, "ket_amd: at most 4 phase bits");
+  const ketk::PairSpec sp = spec_for(sv->n, fixed, mask, 0);
+  HIP_TRY(ketk::launch_phase(sv->stream, sv->precision, sv->data, sp, re, im),
+          "apply_phase");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_apply_x(ket_sv* sv, int target, uint64_t ctrl_mask) {
+  if (int st = check_sv(sv)) return st;
+  if (int st = check_qubit(sv, target)) return st;
+  if (int st = check_mask(sv, ctrl_mask)) return st;
+  if (ctrl_mask & (uint64_t{1} << target))
+    return fail(KET_ERR_ARG, "ket_amd: target is also a control");
+  std::vector<int> fixed = mask_bits(ctrl_mask);
+  if (fixed.size() > 3) return fail(KET_ERR_UNSUPPORTED, "ket_amd: at most 3 controls");
+  fixed.push_back(target);
+  const ketk::PairSpec sp = spec_for(sv->n, fixed, ctrl_mask, uint64_t{1} << target);
+  const double none[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_TRY(ketk::launch_pair(sv->stream, sv->precision, sv->data, sp, ketk::OP_SWAP, none),
+          "apply_x");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_apply_swap(ket_sv* sv, int a, int b, uint64_t ctrl_mask) {
+  if (int st = check_sv(sv)) return st;
+  if (int st = check_qubit(sv, a)) return st;
+  if (int st = check_qubit(sv, b)) return st;
+  if (int st = check_mask(sv, ctrl_mask)) return st;
+  if (a == b || (ctrl_mask & ((uint64_t{1} << a) | (uint64_t{1} << b))))
+    return fail(KET_ERR_ARG, "ket_amd: swap qubits must be distinct from each other and the controls");
+  std::vector<int> fixed = mask_bits(ctrl_mask);
+  if (fixed.size() > 2) return fail(KET_ERR_UNSUPPORTED, "ket_amd: at most 2 controls");
+  fixed.push_back(a);
+  fixed.push_back(b);
+  const uint64_t ma = uint64_t{1} << a, mb = uint64_t{1} << b;
+  const ketk::PairSpec sp = spec_for(sv->n, fixed, ctrl_mask | ma, ma | mb);
+  const double none[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIP_TRY(ketk::launch_pair(sv->stream, sv->precision, sv->data, sp, ketk::OP_SWAP, none),
+          "apply_swap");
+  sv->sample_valid = false;
+  return KET_OK;
+}
+
+int ket_sv_apply_gate(ket_sv* sv, const ket_gate* gate) {
+  if (int st = check_sv(sv)) return st;
+  if (!gate) return fail(KET_ERR_ARG, "ket_amd: null gate");
+  std::string err;
+  if (int st = ketplan::validate(*gate, sv->n, &err)) return fail(st, err);
+  if (!(gate->type >= KET_H && gate->type <= KET_CSWAP)) return KET_OK;
+  ketplan::Plan plan;
+  plan.ops.push_back(ketplan::op_for_gate(*gate, sv->n, sv->precision));
+  return run_plan(sv, plan);
+}
+
+int ket_sv_apply_matrix(ket_sv* sv, int k, const int* qubits, const double* mat) {
+  if (int st = check_sv(sv)) return st;
+  if (k < 1 || k > 6 || k > sv->n) return fail(KET_ERR_ARG, "ket_amd: matrix width must be 1..6 and <= n");
+  if (!qubits || !mat) return fail(KET_ERR_ARG, "ket_amd: null qubits or matrix");
+  for (int i = 0; i < k; ++i) {
+    if (int st = check_qubit(sv, qubits[i])) return st;
+    for (int j = i + 1; j < k; ++j)
+      if (qubits[i] == qubits[j]) return fail(KET_ERR_ARG, "ket_amd: repeated qubit");
+  }
+  ketplan::Plan plan;
+  plan.ops.push_back(ketplan::op_for_matrix(plan, sv->n, sv->precision, k, qubits, mat, true));
+  return run_plan(sv, plan);
+}
+
+int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int fusion,
+                         int block_k, ket_run_stats* stats) {
+  if (int st = check_sv(sv)) return st;
+  if (n_gates && !gates) return fail(KET_ERR_ARG, "ket_amd: null gate list");
+  const auto t0 = std::chrono::steady_clock::now();
+  ketplan::Plan plan;
+  std::string err;
+  if (int st = ketplan::plan_circuit(gates, n_gates, sv->n, sv->precision, fusion, block_k,
+                                     &plan, &err))
+    return fail(st, err);
+  const double plan_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(sv, plan, plan_s, &sv->last);
+  if (stats) *stats = sv->last;
+  return run_plan(sv, plan);
+}
+
+int ket_sv_norm(ket_sv* sv, double* out) {
+  if (int st = check_sv(sv)) return st;
+  if (!out) return fail(KET_ERR_ARG, "ket_amd: null output");
+  HIP_TRY(ketk::launch_norm(sv->stream, sv->precision, sv->data, sv->dim, sv->work, sv->result),
+          "norm");
+  HIP_TRY(hipMemcpyAsync(out, sv->result, sizeof(double), hipMemcpyDeviceToHost, sv->stream),
+          "copy");
+  HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  return KET_OK;
+}
+
+int ket_sv_renormalize(ket_sv* sv) {
+  if (int st = check_sv(sv)) return st;
+  HIP_TRY(ketk::launch_norm(sv->stream, sv->precision, sv->data, sv->dim, sv->work, sv->result),
+          "norm");
+  HIP_TRY(ketk::launch_scale(sv->stream, sv->precision, sv->data, sv->dim, sv->result),
+          "scale");
+  sv->sample_valid = false;
+  HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  return KET_OK;
+}
+
+int ket_sv_sample(ket_sv* sv, const double* u, uint64_t n, uint64_t* out) {
+  if (int st = check_sv(sv)) return st;
+  if (n == 0) return KET_OK;
+  if (!u || !out) return fail(KET_ERR_ARG, "ket_amd: null sample buffers");
+  if (!sv->sample_valid) {
+    if (!sv->sbuf) {
+      HIP_TRY(hipMalloc(&sv->sbuf, ketk::sample_buffer_bytes(sv->dim)), "hipMalloc(sampler)");
+    }
+    ketk::SampleBuffers b;
+    ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
+    HIP_TRY(ketk::launch_sample_prepare(sv->stream, sv->precision, sv->data, sv->dim, b),
+            "sample prepare");
+    sv->sample_valid = true;
+  }
+  ketk::SampleBuffers b;
+  ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
+  const uint64_t step = uint64_t{1} << 22;
+  int st = ensure(&sv->io, &sv->io_cap, (size_t)std::min(n, step) * 16, "hipMalloc(io)");
+  if (st) return st;
+  for (uint64_t o = 0; o < n; o += step) {
+    const uint64_t c = std::min(step, n - o);
+    double* ud = static_cast<double*>(sv->io);
+    uint64_t* od = reinterpret_cast<uint64_t*>(ud + c);
+    HIP_TRY(hipMemcpyAsync(ud, u + o, c * 8, hipMemcpyHostToDevice, sv->stream), "copy u");
+    HIP_TRY(ketk::launch_sample_draw(sv->stream, sv->precision, sv->data, sv->dim, b, ud, c, od),
+            "sample draw");
+    HIP_TRY(hipMemcpyAsync(out + o, od, c * 8, hipMemcpyDeviceToHost, sv->stream), "copy out");
+    HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  }
+  return KET_OK;
+}
+
+int ket_sv_expval_pauli(ket_sv* sv, uint64_t flip, uint64_t sign, int n_y, double* out) {
+  if (int st = check_sv(sv)) return st;
+  if (int st = check_mask(sv, flip | sign)) return st;
+  if (!out) return fail(KET_ERR_ARG, "ket_amd: null output");
+  // i^n_y (src/observable.cpp:62-75)
+  double pr = 1.0, pi = 0.0;
+  switch (((n_y % 4) + 4) % 4) {
+    case 1: pr = 0.0; pi = 1.0; break;
+    case 2: pr = -1.0; pi = 0.0; break;
+    case 3: pr = 0.0; pi = -1.0; break;
+    default: break;
+  }
+  HIP_TRY(ketk::launch_expval(sv->stream, sv->precision, sv->data, sv->dim, flip, sign, pr, pi,
+                              sv->work, sv->result),
+          "expval");
+  HIP_TRY(hipMemcpyAsync(out, sv->result, sizeof(double), hipMemcpyDeviceToHost, sv->stream),
+          "copy");
+  HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  return KET_OK;
+}
+
+int ket_sv_expval_pauli_sum(ket_sv* sv, size_t n_terms, const uint64_t* flip,
+                            const uint64_t* sign, const int* n_y, const double* coeff,
+                            double* out) {
+  if (int st = check_sv(sv)) return st;
+  if (!out) return fail(KET_ERR_ARG, "ket_amd: null output");
+  if (n_terms && (!flip || !sign || !n_y || !coeff))
+    return fail(KET_ERR_ARG, "ket_amd: null term arrays");
+  double total = 0.0;
+  std::vector<uint64_t> dsign;
+  std::vector<double> dcoef;
+  for (size_t t = 0; t < n_terms; ++t) {
+    if (int st = check_mask(sv, flip[t] | sign[t])) return st;
+    if (flip[t] == 0) {
+      dsign.push_back(sign[t]);
+      dcoef.push_back(coeff[t]);
+    } else {
+      double v = 0.0;
+      if (int st = ket_sv_expval_pauli(sv, flip[t], sign[t], n_y[t], &v)) return st;
+      total += coeff[t] * v;
+    }
+  }
+  for (size_t o = 0; o < dsign.size(); o += 64) {
+    const int c = (int)std::min<size_t>(64, dsign.size() - o);
+    int st = ensure(&sv->io, &sv->io_cap, 64 * 16, "hipMalloc(io)");
+    if (st) return st;
+    uint64_t* ds = static_cast<uint64_t*>(sv->io);
+    double* dc = reinterpret_cast<double*>(ds + 64);
+    HIP_TRY(hipMemcpyAsync(ds, dsign.data() + o, c * 8, hipMemcpyHostToDevice, sv->stream), "copy");
+    HIP_TRY(hipMemcpyAsync(dc, dcoef.data() + o, c * 8, hipMemcpyHostToDevice, sv->stream), "copy");
+    HIP_TRY(ketk::launch_expval_diag(sv->stream, sv->precision, sv->data, sv->dim, c, ds, dc,
+                                     sv->work, sv->result),
+            "expval diag");
+    double v = 0.0;
+    HIP_TRY(hipMemcpyAsync(&v, sv->result, 8, hipMemcpyDeviceToHost, sv->stream), "copy");
+    HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+    total += v;
+  }
+  *out = total;
+  return KET_OK;
+}
+
+int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity, double* max_abs_diff) {
+  if (int st = check_sv(a)) return st;
+  if (int st = check_sv(b)) return st;
+  if (a->n != b->n) return fail(KET_ERR_ARG, "ket_amd: fidelity between different sizes");
+  HIP_TRY(hipStreamSynchronize(b->stream), "sync");
+  HIP_TRY(ketk::launch_fidelity(a->stream, a->precision, a->data, b->precision, b->data, a->dim,
+                                a->work, a->result),
+          "fidelity");
+  double r[5];
+  HIP_TRY(hipMemcpyAsync(r, a->result, sizeof r, hipMemcpyDeviceToHost, a->stream), "copy");
+  HIP_TRY(hipStreamSynchronize(a->stream), "sync");
+  if (fidelity) *fidelity = (r[0] * r[0] + r[1] * r[1]) / (r[2] * r[3]);
+  if (max_abs_diff) *max_abs_diff = r[4];
+  return KET_OK;
+}
+
+int ket_sv_event_record(ket_sv* sv, int slot) {
+  if (int st = check_sv(sv)) return st;
+  if (slot < 0 || slot >= 8) return fail(KET_ERR_ARG, "ket_amd: event slot must be 0..7");
+  HIP_TRY(hipEventRecord(sv->ev[slot], sv->stream), "hipEventRecord");
+  return KET_OK;
+}
+
+int ket_sv_event_elapsed(ket_sv* sv, int from, int to, float* ms) {
+  if (int st = check_sv(sv)) return st;
+  if (from < 0 || from >= 8 || to < 0 || to >= 8 || !ms)
+    return fail(KET_ERR_ARG, "ket_amd: bad event slots");
+  HIP_TRY(hipEventSynchronize(sv->ev[to]), "hipEventSynchronize");
+  HIP_TRY(hipEventElapsedTime(ms, sv->ev[from], sv->ev[to]), "hipEventElapsedTime");
+  return KET_OK;
+}
+
+}  // extern "C"

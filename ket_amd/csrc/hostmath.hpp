@@ -1,0 +1,171 @@
+// Host FP64 gate math for the planner.
+//
+// Gate matrices follow ket's per-gate builders (src/simulator.cpp:38-97,
+// 168-200, 322-325) and use the same libm calls, so they are the same
+// doubles. `apply_on_small` runs one gate over a tiny state with ket's kernel
+// arithmetic (src/simulator.cpp:23-254); the Fuser's block matrix
+// (src/simulator.cpp:448-465) is built from it column by column, which makes
+// fused blocks in reference mode equal to ket's value for value. This file is
+// compiled with -ffp-contract=off like the kernels.
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "ket_amd.h"
+
+namespace kethost {
+
+struct cx {
+  double re, im;
+};
+inline cx mul(cx a, cx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+inline cx add(cx a, cx b) { return {a.re + b.re, a.im + b.im}; }
+
+inline int arity(int type) {
+  if (type >= KET_H && type <= KET_U) return 1;
+  if (type >= KET_CH && type <= KET_SWAP) return 2;
+  if (type >= KET_CCX && type <= KET_CSWAP) return 3;
+  return 0;
+}
+inline bool is_unitary(int type) { return arity(type) > 0; }
+
+// The 2x2 matrix ket uses for a single-qubit gate (m00, m01, m10, m11).
+inline void single_matrix(int type, const double* p, cx m[4]) {
+  const cx Z0{0.0, 0.0}, O1{1.0, 0.0};
+  const double r = 1.0 / std::sqrt(2.0);
+  switch (type) {
+    case KET_H: m[0] = {r, 0}; m[1] = {r, 0}; m[2] = {r, 0}; m[3] = {-r, 0}; return;
+    case KET_X: m[0] = Z0; m[1] = O1; m[2] = O1; m[3] = Z0; return;
+    case KET_Y: m[0] = Z0; m[1] = {0.0, -1.0}; m[2] = {0.0, 1.0}; m[3] = Z0; return;
+    case KET_Z: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {-1.0, 0.0}; return;
+    case KET_S: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {0.0, 1.0}; return;
+    case KET_SDG: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {0.0, -1.0}; return;
+    case KET_T: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {r, r}; return;
+    case KET_TDG: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {r, -r}; return;
+    default: break;
+  }
+  const double c = std::cos(p[0] / 2.0), sn = std::sin(p[0] / 2.0);
+  switch (type) {
+    case KET_RX: case KET_CRX:
+      m[0] = {c, 0}; m[1] = {0.0, -sn}; m[2] = {0.0, -sn}; m[3] = {c, 0}; return;
+    case KET_RY: case KET_CRY:
+      m[0] = {c, 0}; m[1] = {-sn, 0}; m[2] = {sn, 0}; m[3] = {c, 0}; return;
+    case KET_RZ: case KET_CRZ:
+      m[0] = {c, -sn}; m[1] = Z0; m[2] = Z0; m[3] = {c, sn}; return;
+    case KET_U: case KET_CU:
+      m[0] = {c, 0.0};
+      m[1] = {-(sn * std::cos(p[2])), -(sn * std::sin(p[2]))};
+      m[2] = {sn * std::cos(p[1]), sn * std::sin(p[1])};
+      m[3] = {c * std::cos(p[1] + p[2]), c * std::sin(p[1] + p[2])};
+      return;
+    default:
+      m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = O1; return;
+  }
+}
+
+// One gate over a small state (size 2^m) with ket's kernel arithmetic; `on`
+// are local qubit indices in the gate's operand order.
+inline void apply_on_small(std::vector<cx>& s, int type, const int* on,
+                           const double* p) {
+  const std::size_t dim = s.size();
+  auto bit = [](int q) { return std::size_t{1} << q; };
+  auto single = [&](std::size_t tm, std::size_t cm, const cx m[4]) {
+    for (std::size_t k = 0; k < dim; ++k) {
+      if ((k & tm) || (k & cm) != cm) continue;
+      const std::size_t k1 = k | tm;
+      const cx a = s[k], b = s[k1];
+      s[k] = add(mul(m[0], a), mul(m[1], b));
+      s[k1] = add(mul(m[2], a), mul(m[3], b));
+    }
+  };
+  cx m[4];
+  switch (type) {
+    case KET_H: case KET_X: case KET_Y: case KET_Z: case KET_S: case KET_SDG:
+    case KET_T: case KET_TDG: case KET_RX: case KET_RY: case KET_RZ: case KET_U:
+      single_matrix(type, p, m);
+      single(bit(on[0]), 0, m);
+      return;
+    case KET_CH: {
+      const double r = 1.0 / std::sqrt(2.0);
+      const std::size_t cm = bit(on[0]), tm = bit(on[1]);
+      for (std::size_t k = 0; k < dim; ++k) {
+        if (!(k & cm) || (k & tm)) continue;
+        const cx a = s[k], b = s[k | tm];
+        s[k] = {(a.re + b.re) * r, (a.im + b.im) * r};
+        s[k | tm] = {(a.re - b.re) * r, (a.im - b.im) * r};
+      }
+      return;
+    }
+    case KET_CX: case KET_CCX: {
+      const int nc = type == KET_CX ? 1 : 2;
+      std::size_t cm = 0;
+      for (int i = 0; i < nc; ++i) cm |= bit(on[i]);
+      const std::size_t tm = bit(on[nc]);
+      for (std::size_t k = 0; k < dim; ++k)
+        if ((k & cm) == cm && !(k & tm)) std::swap(s[k], s[k | tm]);
+      return;
+    }
+    case KET_CY: {
+      const std::size_t cm = bit(on[0]), tm = bit(on[1]);
+      for (std::size_t k = 0; k < dim; ++k) {
+        if (!(k & cm) || (k & tm)) continue;
+        const cx a = s[k], b = s[k | tm];
+        s[k] = mul(cx{0.0, -1.0}, b);
+        s[k | tm] = mul(cx{0.0, 1.0}, a);
+      }
+      return;
+    }
+    case KET_CRX: case KET_CRY: case KET_CRZ: case KET_CU:
+      single_matrix(type, p, m);
+      single(bit(on[1]), bit(on[0]), m);
+      return;
+    case KET_CZ: case KET_CP: {
+      const cx ph = type == KET_CZ ? cx{-1.0, 0.0}
+                                   : cx{std::cos(p[0]), std::sin(p[0])};
+      const std::size_t ma = bit(on[0]), mb = bit(on[1]);
+      for (std::size_t k = 0; k < dim; ++k)
+        if ((k & ma) && (k & mb)) s[k] = mul(s[k], ph);
+      return;
+    }
+    case KET_SWAP: case KET_CSWAP: {
+      const int o = type == KET_SWAP ? 0 : 1;
+      const std::size_t cm = type == KET_SWAP ? 0 : bit(on[0]);
+      const std::size_t ma = bit(on[o]), mb = bit(on[o + 1]);
+      for (std::size_t k = 0; k < dim; ++k) {
+        if ((k & cm) != cm) continue;
+        if ((k & ma) && !(k & mb)) std::swap(s[k], s[(k ^ ma) | mb]);
+      }
+      return;
+    }
+    default:
+      return;
+  }
+}
+
+// Row-major 2^m x 2^m matrix of a gate run on block qubits `block`: column c
+// is the run applied to |c> in the block's local qubit order.
+struct GateRef {
+  int type;
+  int local[3];
+  const double* params;
+};
+inline std::vector<cx> block_matrix(const std::vector<GateRef>& gates, int m) {
+  const std::size_t d = std::size_t{1} << m;
+  std::vector<cx> mat(d * d);
+  std::vector<cx> e(d);
+  for (std::size_t col = 0; col < d; ++col) {
+    for (auto& x : e) x = {0.0, 0.0};
+    e[col] = {1.0, 0.0};
+    for (const GateRef& g : gates) apply_on_small(e, g.type, g.local, g.params);
+    for (std::size_t row = 0; row < d; ++row) mat[row * d + col] = e[row];
+  }
+  return mat;
+}
+
+}  // namespace kethost

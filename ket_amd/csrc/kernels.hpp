@@ -1,0 +1,84 @@
+// Host-side launch interface of the gfx950 kernels (internal to ket_amd).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ketk {
+
+// A state pass that visits a subspace: item i becomes an index by inserting
+// zero bits at pos[0..npos) (ascending), then OR-ing set_mask. Pair kernels
+// pair that index `lo` with lo ^ partner.
+struct PairSpec {
+  uint64_t n_items;
+  int npos;
+  int pos[4];
+  uint64_t set_mask;
+  uint64_t partner;
+};
+
+enum PairOp { OP_DENSE = 0, OP_CH = 1, OP_SWAP = 2 };
+
+// precision: 0 = complex128, 1 = complex64.
+hipError_t launch_fill_basis(hipStream_t st, int precision, void* s,
+                             uint64_t dim, uint64_t index);
+hipError_t launch_pair(hipStream_t st, int precision, void* s,
+                       const PairSpec& spec, int op, const double m[8]);
+hipError_t launch_phase(hipStream_t st, int precision, void* s,
+                        const PairSpec& spec, double re, double im);
+// ket's apply_matrix arithmetic (sequential complex accumulation per row).
+// mat: device pointer to 4^k complex128; k <= 6.
+hipError_t launch_matrix_exact(hipStream_t st, int precision, void* s,
+                               int n, int k, const int* qubits,
+                               const void* mat_dev);
+hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
+                          int src_precision, const void* src, uint64_t count);
+
+// Dense fused block with fused multiply-adds; complex64 runs on MFMA.
+// mat_dev: device matrix in the precision of the state (row-major complex).
+hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
+                        const int* qubits, const void* mat_dev);
+
+// ---- reductions; `work` is device scratch of at least reduce_work_bytes.
+size_t reduce_work_bytes(uint64_t dim);
+hipError_t launch_norm(hipStream_t st, int precision, const void* s,
+                       uint64_t dim, double* work, double* out_dev);
+hipError_t launch_scale(hipStream_t st, int precision, void* s, uint64_t dim,
+                        const double* inv_dev);
+hipError_t launch_expval(hipStream_t st, int precision, const void* s,
+                         uint64_t dim, uint64_t flip, uint64_t sign,
+                         double ph_re, double ph_im, double* work,
+                         double* out_dev);
+// Diagonal Pauli sum: sum_t coeff[t] * (-1)^popcount(x & sign[t]) |a_x|^2.
+// sign/coeff are device arrays of n_terms (<= 64) entries.
+hipError_t launch_expval_diag(hipStream_t st, int precision, const void* s,
+                              uint64_t dim, int n_terms, const uint64_t* sign,
+                              const double* coeff, double* work,
+                              double* out_dev);
+hipError_t launch_fidelity(hipStream_t st, int pa, const void* a, int pb,
+                           const void* b, uint64_t dim, double* work,
+                           double* out_dev /* 5 doubles */);
+
+// ---- exact Born sampling (see sv_sample.hip) ----
+struct SampleBuffers {
+  uint64_t n_chunks;
+  int chunk_log;
+  double* approx;     // n_chunks approximate chunk sums, then exclusive prefix
+  int* e0;            // n_chunks candidate binade base
+  unsigned* flags;    // n_chunks validity of the three candidates
+  uint64_t* msum;     // 3 * n_chunks integer sums
+  double* exact;      // n_chunks + 1 exact sequential prefix at chunk starts
+  int* mode;          // n_chunks: 0..2 candidate used, -1 sequential
+};
+size_t sample_buffer_bytes(uint64_t dim);
+void sample_buffers_carve(void* base, uint64_t dim, SampleBuffers* b);
+hipError_t launch_sample_prepare(hipStream_t st, int precision, const void* s,
+                                 uint64_t dim, const SampleBuffers& b);
+hipError_t launch_sample_draw(hipStream_t st, int precision, const void* s,
+                              uint64_t dim, const SampleBuffers& b,
+                              const double* u_dev, uint64_t n,
+                              uint64_t* out_dev);
+
+}  // namespace ketk

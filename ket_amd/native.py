@@ -1,0 +1,257 @@
+"""ctypes binding of the C ABI in include/ket_amd.h (libket_amd.so).
+
+The library is built in-tree by `make -C ket_amd`; importing this module never
+builds anything and raises if the library is missing, so a GPU run without
+the HIP extension fails loudly instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .gates import GATE_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libket_amd.so")
+
+C128, C64 = 0, 1
+FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED = 0, 1, 2, 3
+
+STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: MemoryError,
+          5: NotImplementedError}
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("gates", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("blocks", ctypes.c_uint64), ("alg_bytes", ctypes.c_double),
+                ("flops", ctypes.c_double), ("plan_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C ket_amd` "
+                "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, u64, dbl, sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64,
+                                 ctypes.c_double, ctypes.c_size_t)
+        P = ctypes.POINTER
+        sigs = {
+            "ket_last_error": ([], ctypes.c_char_p),
+            "ket_device_info": ([ctypes.c_char_p, sz, P(i32), P(u64)], i32),
+            "ket_sv_create": ([i32, i32, P(vp)], i32),
+            "ket_sv_destroy": ([vp], i32),
+            "ket_sv_info": ([vp, P(i32), P(i32)], i32),
+            "ket_sv_set_basis": ([vp, u64], i32),
+            "ket_sv_copy_in": ([vp, u64, u64, vp], i32),
+            "ket_sv_copy_out": ([vp, u64, u64, vp], i32),
+            "ket_sv_copy": ([vp, vp], i32),
+            "ket_sv_sync": ([vp], i32),
+            "ket_sv_apply_1q": ([vp, i32, u64, vp], i32),
+            "ket_sv_apply_phase": ([vp, u64, dbl, dbl], i32),
+            "ket_sv_apply_x": ([vp, i32, u64], i32),
+            "ket_sv_apply_swap": ([vp, i32, i32, u64], i32),
+            "ket_sv_apply_gate": ([vp, vp], i32),
+            "ket_sv_apply_matrix": ([vp, i32, vp, vp], i32),
+            "ket_sv_apply_circuit": ([vp, vp, sz, i32, i32, P(RunStats)], i32),
+            "ket_sv_norm": ([vp, P(dbl)], i32),
+            "ket_sv_renormalize": ([vp], i32),
+            "ket_sv_sample": ([vp, vp, u64, vp], i32),
+            "ket_sv_expval_pauli": ([vp, u64, u64, i32, P(dbl)], i32),
+            "ket_sv_expval_pauli_sum": ([vp, sz, vp, vp, vp, vp, P(dbl)], i32),
+            "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
+            "ket_sv_event_record": ([vp, i32], i32),
+            "ket_sv_event_elapsed": ([vp, i32, i32, P(ctypes.c_float)], i32),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().ket_last_error().decode(errors="replace")
+        raise STATUS.get(rc, RuntimeError)(msg)
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(256)
+    cu, hbm = ctypes.c_int(0), ctypes.c_uint64(0)
+    check(lib().ket_device_info(name, 256, ctypes.byref(cu), ctypes.byref(hbm)))
+    return {"name": name.value.decode(), "compute_units": cu.value, "hbm_bytes": hbm.value}
+
+
+def pauli_masks(pauli: str, n: int):
+    """flip/sign masks and the Y count ket's expval derives from a Pauli string
+    (reference src/observable.cpp:34-60); rightmost character is qubit 0."""
+    if len(pauli) != n:
+        raise ValueError("expval: Pauli string length must equal the number of qubits")
+    flip = sign = n_y = 0
+    for i in range(n):
+        p = pauli[n - 1 - i].upper()
+        b = 1 << i
+        if p == "I":
+            continue
+        if p == "X":
+            flip |= b
+        elif p == "Y":
+            flip |= b
+            sign |= b
+            n_y += 1
+        elif p == "Z":
+            sign |= b
+        else:
+            raise ValueError(f"expval: invalid Pauli character '{pauli[n - 1 - i]}'")
+    return flip, sign, n_y
+
+
+class StateVector:
+    """A 2^n state resident on the GPU (complex128 or complex64)."""
+
+    def __init__(self, n_qubits: int, precision: int = C128):
+        h = ctypes.c_void_p()
+        check(lib().ket_sv_create(int(n_qubits), int(precision), ctypes.byref(h)))
+        self._h = h
+        self.n = int(n_qubits)
+        self.precision = int(precision)
+        self.last_stats: dict | None = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ket_sv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_basis(self, index: int = 0):
+        check(lib().ket_sv_set_basis(self._h, int(index)))
+
+    def load(self, state: np.ndarray):
+        a = np.ascontiguousarray(state, dtype=np.complex128)
+        check(lib().ket_sv_copy_in(self._h, 0, a.size, _p(a)))
+
+    def to_numpy(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        count = (1 << self.n) - offset if count is None else count
+        out = np.empty(count, dtype=np.complex128)
+        check(lib().ket_sv_copy_out(self._h, offset, count, _p(out)))
+        return out
+
+    def copy_from(self, other: "StateVector"):
+        check(lib().ket_sv_copy(self._h, other._h))
+
+    def sync(self):
+        check(lib().ket_sv_sync(self._h))
+
+    def apply_gate_record(self, rec):
+        r = np.ascontiguousarray(rec, dtype=GATE_DTYPE).reshape(1)
+        check(lib().ket_sv_apply_gate(self._h, _p(r)))
+
+    def apply_1q(self, target, m, ctrl_mask=0):
+        mm = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(4))
+        check(lib().ket_sv_apply_1q(self._h, int(target), int(ctrl_mask), _p(mm)))
+
+    def apply_phase(self, mask, phase):
+        check(lib().ket_sv_apply_phase(self._h, int(mask), float(np.real(phase)),
+                                       float(np.imag(phase))))
+
+    def apply_x(self, target, ctrl_mask=0):
+        check(lib().ket_sv_apply_x(self._h, int(target), int(ctrl_mask)))
+
+    def apply_swap(self, a, b, ctrl_mask=0):
+        check(lib().ket_sv_apply_swap(self._h, int(a), int(b), int(ctrl_mask)))
+
+    def apply_matrix(self, qubits, mat):
+        q = np.ascontiguousarray(qubits, dtype=np.int32)
+        m = np.ascontiguousarray(mat, dtype=np.complex128)
+        check(lib().ket_sv_apply_matrix(self._h, len(q), _p(q), _p(m)))
+
+    def apply_records(self, rec: np.ndarray, fusion: int = FUSE_REFERENCE,
+                      block_k: int = 5) -> dict:
+        r = np.ascontiguousarray(rec, dtype=GATE_DTYPE)
+        st = RunStats()
+        check(lib().ket_sv_apply_circuit(self._h, _p(r), len(r), int(fusion), int(block_k),
+                                         ctypes.byref(st)))
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def apply_circuit(self, circuit, fusion: int = FUSE_REFERENCE, block_k: int = 5) -> dict:
+        return self.apply_records(circuit.records(), fusion, block_k)
+
+    def norm(self) -> float:
+        v = ctypes.c_double(0)
+        check(lib().ket_sv_norm(self._h, ctypes.byref(v)))
+        return v.value
+
+    def renormalize(self):
+        check(lib().ket_sv_renormalize(self._h))
+
+    def sample(self, u: np.ndarray) -> np.ndarray:
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(uu.size, dtype=np.uint64)
+        check(lib().ket_sv_sample(self._h, _p(uu), uu.size, _p(out)))
+        return out
+
+    def expval(self, pauli: str) -> float:
+        flip, sign, n_y = pauli_masks(pauli, self.n)
+        v = ctypes.c_double(0)
+        check(lib().ket_sv_expval_pauli(self._h, flip, sign, n_y, ctypes.byref(v)))
+        return v.value
+
+    def expval_sum(self, terms) -> float:
+        """terms: iterable of (coeff, pauli)."""
+        terms = list(terms)
+        fl = np.zeros(len(terms), np.uint64)
+        sg = np.zeros(len(terms), np.uint64)
+        ny = np.zeros(len(terms), np.int32)
+        cf = np.zeros(len(terms), np.float64)
+        for i, (c, p) in enumerate(terms):
+            fl[i], sg[i], ny[i] = pauli_masks(p, self.n)
+            cf[i] = c
+        v = ctypes.c_double(0)
+        check(lib().ket_sv_expval_pauli_sum(self._h, len(terms), _p(fl), _p(sg), _p(ny),
+                                            _p(cf), ctypes.byref(v)))
+        return v.value
+
+    def fidelity(self, other: "StateVector"):
+        f, d = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().ket_sv_fidelity(self._h, other._h, ctypes.byref(f), ctypes.byref(d)))
+        return f.value, d.value
+
+    def event_record(self, slot: int):
+        check(lib().ket_sv_event_record(self._h, slot))
+
+    def event_elapsed(self, a: int, b: int) -> float:
+        ms = ctypes.c_float(0)
+        check(lib().ket_sv_event_elapsed(self._h, a, b, ctypes.byref(ms)))
+        return ms.value

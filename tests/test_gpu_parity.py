@@ -1,0 +1,207 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+complex128 per-gate passes and ket's Fuser rule must equal the oracle value
+for value (IEEE ==, which lets only the sign of a zero differ); dense fused
+blocks (fused multiply-adds) within 1e-12; complex64 within 1-F <= 1e-6.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import random_state
+from ket_amd import native as N
+from ket_amd import workloads as W
+from ket_amd.gates import ARITY, NPARAMS, Circuit
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_run(c, fusion, precision=N.C128, k=5, state=None):
+    with N.StateVector(c.n_qubits, precision) as sv:
+        if state is not None:
+            sv.load(state)
+        sv.apply_circuit(c, fusion, k)
+        return sv.to_numpy()
+
+
+def fid(a, b):
+    return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+GATES = [g for g in ARITY]
+
+
+@pytest.mark.parametrize("name", GATES)
+def test_each_gate_every_placement_exact(name):
+    n = 7
+    k = ARITY[name]
+    rng = np.random.default_rng(hash(name) % 1000)
+    placements = [p for p in itertools.permutations(range(n), k)]
+    rng.shuffle(placements)
+    placements = placements[:24] + [tuple(range(k)), tuple(range(n - k, n))]
+    s0 = random_state(n, 3)
+    for qs in placements:
+        c = Circuit(n)
+        params = list(rng.uniform(0, 2 * np.pi, NPARAMS.get(name, 0)))
+        getattr(c, name)(*qs, *params)
+        want = O.run(c, fuse=False, state=s0)
+        got = gpu_run(c, N.FUSE_NONE, state=s0)
+        assert np.array_equal(got, want), (name, qs, np.abs(got - want).max())
+
+
+CIRCUITS = {
+    "mixed12": lambda: W.random_mixed(12, 14, 7),
+    "brick14": lambda: W.brickwork(14, 10, seed=2),
+    "qft12": lambda: W.qft_u(12, seed=3),
+    "hea10": lambda: W.hea(10, 8, seed=4),
+    "bwm13": lambda: W.brickwork_mixed(13, 12, seed=5),
+}
+
+
+@pytest.mark.parametrize("cname", sorted(CIRCUITS))
+@pytest.mark.parametrize("fusion", [N.FUSE_NONE, N.FUSE_REFERENCE])
+def test_circuit_exact(cname, fusion):
+    c = CIRCUITS[cname]()
+    want = O.run(c, fuse=(fusion == N.FUSE_REFERENCE))
+    got = gpu_run(c, fusion)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("cname", sorted(CIRCUITS))
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6])
+def test_blocks_complex128(cname, k):
+    c = CIRCUITS[cname]()
+    want = O.run(c)
+    got = gpu_run(c, N.FUSE_BLOCKS, k=k)
+    assert np.abs(got - want).max() <= 1e-12
+    assert 1 - fid(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("cname", sorted(CIRCUITS))
+@pytest.mark.parametrize("fusion,k", [(N.FUSE_NONE, 5), (N.FUSE_REFERENCE, 5),
+                                      (N.FUSE_BLOCKS, 4), (N.FUSE_BLOCKS, 5)])
+def test_complex64(cname, fusion, k):
+    c = CIRCUITS[cname]()
+    want = O.run(c)
+    got = gpu_run(c, fusion, N.C64, k)
+    assert 1 - fid(got, want) <= 1e-6
+    assert np.abs(got - want).max() <= 1e-5
+
+
+def test_apply_matrix_exact():
+    n = 9
+    rng = np.random.default_rng(11)
+    s0 = random_state(n, 12)
+    for k in (1, 2, 3, 4, 5, 6):
+        q = [int(x) for x in rng.permutation(n)[:k]]
+        a = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        m, _ = np.linalg.qr(a)
+        want = O.apply_matrix(s0, q, m)
+        with N.StateVector(n) as sv:
+            sv.load(s0)
+            sv.apply_matrix(q, m)
+            got = sv.to_numpy()
+        assert np.array_equal(got, want), (k, q)
+
+
+def test_sampling_matches_sequential_scan_config1():
+    c = W.brickwork(20, 20, seed=0)
+    state = O.run(c)
+    u = O.uniforms(1, 10_000)
+    want = np.array([O.measure_u(state, x) for x in u[:2000]], dtype=np.uint64)
+    with N.StateVector(20) as sv:
+        sv.apply_circuit(c, N.FUSE_REFERENCE)
+        got = sv.sample(u)
+    assert np.array_equal(got[:2000], want)
+    want_all = O.sample(state, 1, 10_000)
+    assert np.array_equal(got, want_all)
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 8, 12, 13])
+def test_sampling_small_and_edge(n):
+    rng = np.random.default_rng(n)
+    s = random_state(n, 100 + n) if n else np.ones(1, np.complex128)
+    u = np.concatenate([rng.uniform(size=500), [0.0, np.nextafter(1.0, 0.0), 0.999999999999]])
+    with N.StateVector(n) as sv:
+        sv.load(s)
+        got = sv.sample(u)
+    want = np.array([O.measure_u(s, x) for x in u], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_sampling_boundaries_exact():
+    # u drawn exactly at (and one ulp around) the sequential cumulative sums.
+    n = 14
+    s = random_state(n, 77)
+    p = s.real ** 2 + s.imag ** 2
+    cum = np.empty_like(p)
+    acc = 0.0
+    for i, x in enumerate(p):
+        acc += x
+        cum[i] = acc
+    rng = np.random.default_rng(5)
+    idx = rng.integers(0, (1 << n) - 1, 300)
+    u = np.concatenate([cum[idx], np.nextafter(cum[idx], 0), np.nextafter(cum[idx], 2)])
+    u = u[u < 1.0]
+    with N.StateVector(n) as sv:
+        sv.load(s)
+        got = sv.sample(u)
+    want = np.array([O.measure_u(s, x) for x in u], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_sampling_complex64_uses_fp64_probabilities():
+    c = W.brickwork(16, 6, seed=9)
+    with N.StateVector(16, N.C64) as sv:
+        sv.apply_circuit(c, N.FUSE_BLOCKS, 5)
+        s = sv.to_numpy()  # float amplitudes widened exactly
+        u = O.uniforms(3, 3000)
+        got = sv.sample(u)
+    want = np.array([O.measure_u(s, x) for x in u], dtype=np.uint64)
+    assert np.array_equal(got, want)
+
+
+def test_expval_matches_oracle():
+    c = W.random_mixed(11, 10, 21)
+    s = O.run(c)
+    rng = np.random.default_rng(1)
+    paulis = ["".join(rng.choice(list("IXYZ"), 11)) for _ in range(12)]
+    paulis += ["Z" * 11, "I" * 11, "xyzxyzxyzxy"]
+    with N.StateVector(11) as sv:
+        sv.apply_circuit(c)
+        for p in paulis:
+            assert abs(sv.expval(p) - O.expval(s, p)) <= 1e-12, p
+        terms = [(0.5, p) for p in paulis] + [(0.25, q) for q in W.zz_chain(11)]
+        want = sum(cf * O.expval(s, p) for cf, p in terms)
+        assert abs(sv.expval_sum(terms) - want) <= 1e-12
+
+
+def test_norm_fidelity_renormalize():
+    n = 15
+    s = random_state(n, 8) * 1.7
+    with N.StateVector(n) as a, N.StateVector(n, N.C64) as b:
+        a.load(s)
+        assert abs(a.norm() - np.vdot(s, s).real) <= 1e-12 * 3
+        b.copy_from(a)
+        f, d = a.fidelity(b)
+        assert 1 - f <= 1e-12 and d <= 1e-6
+        a.renormalize()
+        assert abs(a.norm() - 1.0) <= 1e-14
+
+
+def test_errors_are_statuses():
+    with N.StateVector(4) as sv:
+        with pytest.raises(IndexError):
+            sv.apply_x(4)
+        with pytest.raises(ValueError):
+            sv.apply_swap(1, 1)
+        c = Circuit(6)
+        c.h(5)
+        with pytest.raises(IndexError):
+            sv.apply_circuit(c)
+    with pytest.raises(MemoryError):
+        N.StateVector(40)
+    with pytest.raises(ValueError):
+        N.StateVector(41)
