@@ -156,6 +156,34 @@ int ket_sv_expval_pauli_sum(ket_sv* sv, size_t n_terms, const uint64_t* flip,
 int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity,
                     double* max_abs_diff);
 
+/* ---- per-launch profile (the cost model's training data) ---- */
+typedef enum ket_launch_kind {
+  KET_LAUNCH_DENSE1Q = 0,  /* 2x2 on a target, optional controls */
+  KET_LAUNCH_CH = 1,       /* ket's controlled-H arithmetic */
+  KET_LAUNCH_PERMUTE = 2,  /* X / CX / CCX / SWAP / CSWAP exchanges */
+  KET_LAUNCH_PHASE = 3,    /* diagonal phase on a bit mask */
+  KET_LAUNCH_MATRIX = 4,   /* ket's exact apply_matrix arithmetic */
+  KET_LAUNCH_BLOCK = 5     /* fused dense 2^k block */
+} ket_launch_kind;
+
+typedef struct ket_launch_record {
+  int32_t kind;          /* ket_launch_kind */
+  int32_t k;             /* block width (0 for per-gate passes) */
+  int32_t n_gates;       /* unitary gates the pass applies */
+  int32_t pad;
+  uint64_t qubit_mask;   /* qubits the pass acts on */
+  double alg_bytes;      /* algorithmic HBM bytes (128-byte line model) */
+  double flops;
+  double ms;             /* device time between events around the launch */
+} ket_launch_record;
+
+/* Plans and runs the circuit like ket_sv_apply_circuit, timing every launch
+ * with HIP events on the state's stream. Writes min(cap, launches) records;
+ * *n_launches gets the total. */
+int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
+                           int fusion, int block_k, ket_launch_record* out,
+                           size_t cap, size_t* n_launches);
+
 /* ---- timing on the state's stream ---- */
 /* Record event slot (0..7) on the state's stream; elapsed ms between two. */
 int ket_sv_event_record(ket_sv* sv, int slot);

@@ -484,6 +484,83 @@ int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int 
   return run_plan(sv, plan);
 }
 
+int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int fusion,
+                           int block_k, ket_launch_record* out, size_t cap,
+                           size_t* n_launches) {
+  if (int st = check_sv(sv)) return st;
+  if (n_gates && !gates) return fail(KET_ERR_ARG, "ket_amd: null gate list");
+  if (cap && !out) return fail(KET_ERR_ARG, "ket_amd: null record buffer");
+  ketplan::Plan plan;
+  std::string err;
+  if (int st = ketplan::plan_circuit(gates, n_gates, sv->n, sv->precision, fusion, block_k,
+                                     &plan, &err))
+    return fail(st, err);
+  fill_stats(sv, plan, 0.0, &sv->last);
+  const size_t nl = plan.ops.size();
+  if (n_launches) *n_launches = nl;
+  std::vector<hipEvent_t> ev(nl + 1, nullptr);
+  auto destroy = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  for (auto& e : ev) {
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      destroy();
+      return hip_fail(r, "hipEventCreate");
+    }
+  }
+  ketplan::Plan empty_ops = plan;
+  empty_ops.ops.clear();
+  int st = run_plan(sv, empty_ops);  // uploads the pool only
+  const unsigned char* pool_dev = static_cast<const unsigned char*>(sv->pool);
+  for (size_t i = 0; st == KET_OK && i < nl; ++i) {
+    if (hipEventRecord(ev[i], sv->stream) != hipSuccess) st = hip_fail(hipGetLastError(), "event");
+    if (st == KET_OK) st = launch_op(sv, plan.ops[i], pool_dev);
+  }
+  if (st == KET_OK && hipEventRecord(ev[nl], sv->stream) != hipSuccess)
+    st = hip_fail(hipGetLastError(), "event");
+  if (st == KET_OK && hipEventSynchronize(ev[nl]) != hipSuccess)
+    st = hip_fail(hipGetLastError(), "event sync");
+  const double sweep = 2.0 * (double)amp_bytes(sv->precision) * (double)sv->dim;
+  for (size_t i = 0; st == KET_OK && i < nl && i < cap; ++i) {
+    const ketplan::Op& o = plan.ops[i];
+    ket_launch_record r{};
+    switch (o.kind) {
+      case ketplan::Kind::Pair:
+        r.kind = o.pair_op == ketk::OP_DENSE ? KET_LAUNCH_DENSE1Q
+                 : o.pair_op == ketk::OP_CH  ? KET_LAUNCH_CH
+                                             : KET_LAUNCH_PERMUTE;
+        for (int b = 0; b < o.spec.npos; ++b) r.qubit_mask |= uint64_t{1} << o.spec.pos[b];
+        break;
+      case ketplan::Kind::Phase:
+        r.kind = KET_LAUNCH_PHASE;
+        r.qubit_mask = o.spec.set_mask;
+        break;
+      case ketplan::Kind::MatrixExact:
+      case ketplan::Kind::Block:
+        r.kind = o.kind == ketplan::Kind::Block ? KET_LAUNCH_BLOCK : KET_LAUNCH_MATRIX;
+        r.k = o.k;
+        for (int b = 0; b < o.k; ++b) r.qubit_mask |= uint64_t{1} << o.qubits[b];
+        break;
+    }
+    r.n_gates = o.n_gates;
+    r.alg_bytes = sweep * o.line_fraction;
+    double touched = (double)sv->dim;
+    if (o.kind == ketplan::Kind::Phase) touched = (double)o.spec.n_items;
+    else if (o.kind == ketplan::Kind::Pair) touched = 2.0 * (double)o.spec.n_items;
+    r.flops = o.flops_per_amp * touched;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess)
+      st = hip_fail(hipGetLastError(), "hipEventElapsedTime");
+    r.ms = ms;
+    out[i] = r;
+  }
+  destroy();
+  sv->sample_valid = false;
+  return st;
+}
+
 int ket_sv_norm(ket_sv* sv, double* out) {
   if (int st = check_sv(sv)) return st;
   if (!out) return fail(KET_ERR_ARG, "ket_amd: null output");

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cmath>
+#include <math.h>
 #include <cstddef>
 #include <cstdint>
 #include <utility>
@@ -22,6 +23,20 @@ namespace kethost {
 struct cx {
   double re, im;
 };
+
+// cos and sin of one argument from glibc's sincos(). ket is built with g++,
+// which merges each std::cos(x)/std::sin(x) pair of the same argument into
+// one sincos() call; sincos can differ from separate sin() and cos() calls in
+// the last bit, so the planner calls it explicitly to get ket's doubles
+// whatever compiler builds this file.
+struct CosSin {
+  double c, s;
+};
+inline CosSin cos_sin(double x) {
+  CosSin r;
+  ::sincos(x, &r.s, &r.c);
+  return r;
+}
 inline cx mul(cx a, cx b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
@@ -50,7 +65,8 @@ inline void single_matrix(int type, const double* p, cx m[4]) {
     case KET_TDG: m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = {r, -r}; return;
     default: break;
   }
-  const double c = std::cos(p[0] / 2.0), sn = std::sin(p[0] / 2.0);
+  const CosSin h = cos_sin(p[0] / 2.0);
+  const double c = h.c, sn = h.s;
   switch (type) {
     case KET_RX: case KET_CRX:
       m[0] = {c, 0}; m[1] = {0.0, -sn}; m[2] = {0.0, -sn}; m[3] = {c, 0}; return;
@@ -59,11 +75,14 @@ inline void single_matrix(int type, const double* p, cx m[4]) {
     case KET_RZ: case KET_CRZ:
       m[0] = {c, -sn}; m[1] = Z0; m[2] = Z0; m[3] = {c, sn}; return;
     case KET_U: case KET_CU:
+    {
+      const CosSin l = cos_sin(p[2]), f = cos_sin(p[1]), fl = cos_sin(p[1] + p[2]);
       m[0] = {c, 0.0};
-      m[1] = {-(sn * std::cos(p[2])), -(sn * std::sin(p[2]))};
-      m[2] = {sn * std::cos(p[1]), sn * std::sin(p[1])};
-      m[3] = {c * std::cos(p[1] + p[2]), c * std::sin(p[1] + p[2])};
+      m[1] = {-(sn * l.c), -(sn * l.s)};
+      m[2] = {sn * f.c, sn * f.s};
+      m[3] = {c * fl.c, c * fl.s};
       return;
+    }
     default:
       m[0] = O1; m[1] = Z0; m[2] = Z0; m[3] = O1; return;
   }
@@ -126,8 +145,8 @@ inline void apply_on_small(std::vector<cx>& s, int type, const int* on,
       single(bit(on[1]), bit(on[0]), m);
       return;
     case KET_CZ: case KET_CP: {
-      const cx ph = type == KET_CZ ? cx{-1.0, 0.0}
-                                   : cx{std::cos(p[0]), std::sin(p[0])};
+      const CosSin t = cos_sin(p[0]);
+      const cx ph = type == KET_CZ ? cx{-1.0, 0.0} : cx{t.c, t.s};
       const std::size_t ma = bit(on[0]), mb = bit(on[1]);
       for (std::size_t k = 0; k < dim; ++k)
         if ((k & ma) && (k & mb)) s[k] = mul(s[k], ph);

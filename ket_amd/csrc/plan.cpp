@@ -168,8 +168,10 @@ Op op_for_gate(const ket_gate& g, int n, int precision) {
     case KET_CZ:
       return phase_op(n, L, {q[0], q[1]}, cx{-1.0, 0.0});
     case KET_CP:
-      return phase_op(n, L, {q[0], q[1]},
-                      cx{std::cos(g.params[0]), std::sin(g.params[0])});
+    {
+      const kethost::CosSin t = kethost::cos_sin(g.params[0]);
+      return phase_op(n, L, {q[0], q[1]}, cx{t.c, t.s});
+    }
     case KET_SWAP:
       return swap_op(n, L, q[0], q[1], {});
     case KET_CCX:

@@ -23,6 +23,16 @@ STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: MemoryError,
           5: NotImplementedError}
 
 
+class LaunchRecord(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("k", ctypes.c_int32), ("n_gates", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("qubit_mask", ctypes.c_uint64),
+                ("alg_bytes", ctypes.c_double), ("flops", ctypes.c_double),
+                ("ms", ctypes.c_double)]
+
+
+LAUNCH_KINDS = ["dense1q", "ch", "permute", "phase", "matrix", "block"]
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("blocks", ctypes.c_uint64), ("alg_bytes", ctypes.c_double),
@@ -70,6 +80,7 @@ def lib():
             "ket_sv_expval_pauli": ([vp, u64, u64, i32, P(dbl)], i32),
             "ket_sv_expval_pauli_sum": ([vp, sz, vp, vp, vp, vp, P(dbl)], i32),
             "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
+            "ket_sv_profile_circuit": ([vp, vp, sz, i32, i32, vp, sz, P(sz)], i32),
             "ket_sv_event_record": ([vp, i32], i32),
             "ket_sv_event_elapsed": ([vp, i32, i32, P(ctypes.c_float)], i32),
         }
@@ -207,6 +218,23 @@ class StateVector:
 
     def apply_circuit(self, circuit, fusion: int = FUSE_REFERENCE, block_k: int = 5) -> dict:
         return self.apply_records(circuit.records(), fusion, block_k)
+
+    def profile_circuit(self, circuit, fusion: int = FUSE_NONE, block_k: int = 5):
+        """Runs the circuit, timing each launch; returns a list of dicts."""
+        r = np.ascontiguousarray(circuit.records(), dtype=GATE_DTYPE)
+        cap = 4 * len(r) + 16
+        buf = (LaunchRecord * cap)()
+        nl = ctypes.c_size_t(0)
+        check(lib().ket_sv_profile_circuit(self._h, _p(r), len(r), int(fusion), int(block_k),
+                                           ctypes.cast(buf, ctypes.c_void_p), cap,
+                                           ctypes.byref(nl)))
+        out = []
+        for i in range(min(nl.value, cap)):
+            x = buf[i]
+            out.append({"kind": LAUNCH_KINDS[x.kind], "k": x.k, "n_gates": x.n_gates,
+                        "qubit_mask": int(x.qubit_mask), "alg_bytes": x.alg_bytes,
+                        "flops": x.flops, "ms": x.ms})
+        return out
 
     def norm(self) -> float:
         v = ctypes.c_double(0)

@@ -6,6 +6,7 @@
  * rounded one operation at a time, and this file must round the same way.
  * Citations are file:line into the reference source tree.
  */
+#define _GNU_SOURCE
 #include "ket_oracle.h"
 
 #include <math.h>
@@ -34,6 +35,16 @@ static inline cx cadd(cx a, cx b) {
 static inline cx mk(double re, double im) {
   cx r = {re, im};
   return r;
+}
+
+/* The reference is built with g++, which turns each cos(x)/sin(x) pair of
+ * one argument into a single glibc sincos() call; sincos can differ from
+ * separate sin() and cos() in the last bit, so the oracle calls it
+ * explicitly (returns (cos x, sin x)). */
+static inline cx cos_sin(double x) {
+  double s, c;
+  sincos(x, &s, &c);
+  return mk(c, s);
 }
 
 /* ---- parallel_for: src/parallelism.cpp:25 (threshold), 73-79 (contiguous
@@ -238,27 +249,32 @@ static int apply_gate_on(cx* s, size_t dim, int type, const int* on,
       return 0;
     }
     case OR_RX: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       apply_single(s, dim, on[0], mk(c, 0), mk(0.0, -sn), mk(0.0, -sn),
                    mk(c, 0));
       return 0;
     }
     case OR_RY: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       apply_single(s, dim, on[0], mk(c, 0), mk(-sn, 0), mk(sn, 0), mk(c, 0));
       return 0;
     }
     case OR_RZ: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       apply_single(s, dim, on[0], mk(c, -sn), Z0, Z0, mk(c, sn));
       return 0;
     }
     case OR_U: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       /* -polar(sn, lambda), polar(sn, phi), polar(c, phi + lambda) */
-      const cx m01 = mk(-(sn * cos(p[2])), -(sn * sin(p[2])));
-      const cx m10 = mk(sn * cos(p[1]), sn * sin(p[1]));
-      const cx m11 = mk(c * cos(p[1] + p[2]), c * sin(p[1] + p[2]));
+      const cx l = cos_sin(p[2]), f = cos_sin(p[1]), fl = cos_sin(p[1] + p[2]);
+      const cx m01 = mk(-(sn * l.re), -(sn * l.im));
+      const cx m10 = mk(sn * f.re, sn * f.im);
+      const cx m11 = mk(c * fl.re, c * fl.im);
       apply_single(s, dim, on[0], mk(c, 0.0), m01, m10, m11);
       return 0;
     }
@@ -279,35 +295,40 @@ static int apply_gate_on(cx* s, size_t dim, int type, const int* on,
                  Z0, Z0, Z0);
       return 0;
     case OR_CRX: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       run_kernel(k_csingle, s, dim, BIT(on[0]), BIT(on[1]), 0, mk(c, 0),
                  mk(0.0, -sn), mk(0.0, -sn), mk(c, 0));
       return 0;
     }
     case OR_CRY: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       run_kernel(k_csingle, s, dim, BIT(on[0]), BIT(on[1]), 0, mk(c, 0),
                  mk(-sn, 0), mk(sn, 0), mk(c, 0));
       return 0;
     }
     case OR_CRZ: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
       run_kernel(k_csingle, s, dim, BIT(on[0]), BIT(on[1]), 0, mk(c, -sn),
                  Z0, Z0, mk(c, sn));
       return 0;
     }
     case OR_CU: {
-      const double c = cos(p[0] / 2.0), sn = sin(p[0] / 2.0);
-      const cx m01 = mk(-(sn * cos(p[2])), -(sn * sin(p[2])));
-      const cx m10 = mk(sn * cos(p[1]), sn * sin(p[1]));
-      const cx m11 = mk(c * cos(p[1] + p[2]), c * sin(p[1] + p[2]));
+      const cx h = cos_sin(p[0] / 2.0);
+      const double c = h.re, sn = h.im;
+      const cx l = cos_sin(p[2]), f = cos_sin(p[1]), fl = cos_sin(p[1] + p[2]);
+      const cx m01 = mk(-(sn * l.re), -(sn * l.im));
+      const cx m10 = mk(sn * f.re, sn * f.im);
+      const cx m11 = mk(c * fl.re, c * fl.im);
       run_kernel(k_csingle, s, dim, BIT(on[0]), BIT(on[1]), 0, mk(c, 0.0),
                  m01, m10, m11);
       return 0;
     }
     case OR_CP:
-      run_kernel(k_cphase, s, dim, BIT(on[0]), BIT(on[1]), 0,
-                 mk(cos(p[0]), sin(p[0])), Z0, Z0, Z0);
+      run_kernel(k_cphase, s, dim, BIT(on[0]), BIT(on[1]), 0, cos_sin(p[0]),
+                 Z0, Z0, Z0);
       return 0;
     case OR_SWAP:
       run_kernel(k_swap, s, dim, BIT(on[0]), BIT(on[1]), 0, Z0, Z0, Z0, Z0);

@@ -45,6 +45,17 @@ def closed_form_circuits():
         c.cx(q, q + 1); c.rz(q + 1, 0.3); c.cx(q, q + 1); c.cz(q, q + 1); c.cp(q + 1, q, 0.7)
     c.ccx(0, 1, 2); c.cswap(0, 1, 2); c.ccx(0, 1, 2)
     out["fuser_runs5"] = c
+    # Many random angles through every parametrised gate: pins the exact
+    # libm path ket's matrices take (sin/cos pairs via sincos).
+    rng = np.random.default_rng(31)
+    c = Circuit(3)
+    for _ in range(400):
+        a, b, d = (float(x) for x in rng.uniform(0, 2 * math.pi, 3))
+        q = [int(x) for x in rng.permutation(3)]
+        c.u(q[0], a, b, d); c.rx(q[1], a); c.ry(q[2], b); c.rz(q[0], d)
+        c.cu(q[0], q[1], a, b, d); c.crx(q[1], q[2], b); c.cry(q[2], q[0], d)
+        c.crz(q[0], q[2], a); c.cp(q[1], q[0], d)
+    out["angles3"] = c
     out["qft10"] = W.qft_u(10, seed=3)
     out["brick10"] = W.brickwork(10, 8, seed=4)
     out["mixed10"] = W.random_mixed(10, 12, seed=5)

@@ -69,6 +69,31 @@ def test_circuit_exact(cname, fusion):
     assert np.array_equal(got, want), np.abs(got - want).max()
 
 
+def test_golden_states_from_reference_exact():
+    """Every small golden circuit, run by the GPU with ket's Fuser rule in
+    complex128, equals the state the reference library wrote."""
+    import os
+    from test_oracle import GOLD, circuit_from_text
+    g = np.load(GOLD, allow_pickle=False)
+    names = sorted({k.split("__")[0] for k in g.files if k.endswith("__state")})
+    for name in names:
+        c = circuit_from_text(g[f"{name}__text"].tobytes().decode())
+        got = gpu_run(c, N.FUSE_REFERENCE)
+        assert np.array_equal(got, g[f"{name}__state"]), name
+
+
+def test_golden_config1_from_reference():
+    from test_oracle import GOLD, circuit_from_text
+    g = np.load(GOLD, allow_pickle=False)
+    c = circuit_from_text(g["brick20__text"].tobytes().decode())
+    with N.StateVector(20) as sv:
+        sv.apply_circuit(c, N.FUSE_REFERENCE)
+        s = sv.to_numpy()
+        shots = sv.sample(O.uniforms(1, 10_000))
+    assert np.array_equal(s[g["brick20__idx"]], g["brick20__amps"])
+    assert np.array_equal(shots.astype(np.uint32), g["brick20__shots_seed1"])
+
+
 @pytest.mark.parametrize("cname", sorted(CIRCUITS))
 @pytest.mark.parametrize("k", [2, 3, 4, 5, 6])
 def test_blocks_complex128(cname, k):
