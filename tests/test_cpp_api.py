@@ -1,0 +1,71 @@
+"""The ket:: C++ API: the same test program and the same driver source built
+against the reference library (CPU, checks the tests themselves) and against
+this repository's GPU backend (must agree with the reference)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from ket_amd import workloads as W
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GPU_TEST = os.path.join(ROOT, "ket_amd", "test_api")
+GPU_DRIVER = os.path.join(ROOT, "ket_amd", "ket_driver")
+REF_TEST = os.path.join(ROOT, "oracle", "_ref", "test_api")
+
+
+def run(cmd, **kw):
+    return subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=300, **kw)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TEST), reason="oracle/_ref not built")
+def test_api_suite_passes_on_reference():
+    r = run([REF_TEST])
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_api_suite_passes_on_gpu_backend():
+    r = run([GPU_TEST])
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def _write(c, d):
+    p = os.path.join(d, "c.txt")
+    with open(p, "w") as f:
+        f.write(c.to_text())
+    return p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 2])
+def test_driver_run_and_sample_match_reference(seed):
+    c = W.random_mixed(14, 12, seed)
+    c.measure_all()
+    with tempfile.TemporaryDirectory() as d:
+        cp = _write(c, d)
+        out = os.path.join(d, "s.bin")
+        r = run([GPU_DRIVER, "run", cp, out])
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(out, dtype=np.complex128)
+        want = O.ref_run(c) if O.ref_available() else O.run(c)
+        assert np.array_equal(got, want)
+        r = run([GPU_DRIVER, "sample", cp, seed, 3000, out])
+        assert r.returncode == 0, r.stderr
+        shots = np.fromfile(out, dtype=np.uint64)
+        want_s = O.ref_sample(c, seed, 3000) if O.ref_available() else O.sample(want, seed, 3000)
+        assert np.array_equal(shots, want_s)
+        r = run([GPU_DRIVER, "sample-calls", cp, seed, 50, out])
+        assert r.returncode == 0, r.stderr
+        assert np.array_equal(np.fromfile(out, dtype=np.uint64), want_s[:50])
+        paulis = ["Z" * 14, "XY" * 7, "I" * 13 + "Y"]
+        r = run([GPU_DRIVER, "expval", cp] + paulis)
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.strip().split("\n")
+        assert lines[0] == "state vector"
+        for p, v in zip(paulis, lines[1:]):
+            assert abs(float(v) - O.expval(want, p)) <= 1e-12
