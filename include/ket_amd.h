@@ -184,6 +184,23 @@ int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
                            int fusion, int block_k, ket_launch_record* out,
                            size_t cap, size_t* n_launches);
 
+/* ---- launch shape of the per-gate passes (the autotuner's search space,
+ *      process-wide; no counterpart in the reference, whose CPU loops have
+ *      no launch shape) ---- */
+typedef struct ket_tuning {
+  int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 2, 4, 8 */
+  int32_t nontemporal;      /* 1: streaming (nontemporal) loads and stores */
+  int32_t blocks_per_cu;    /* 0: one block per 256*items items; else cap the
+                               grid at 256*blocks_per_cu blocks (grid-stride) */
+  int32_t pad;
+} ket_tuning;
+int ket_set_tuning(const ket_tuning* t);
+int ket_get_tuning(ket_tuning* t);
+/* Achievable HBM bandwidth on this device: a plain copy of `bytes` between
+ * two device buffers, `reps` times; *gbps = 2 * bytes * reps / time / 1e9
+ * (read plus write). Allocates and frees the two buffers. */
+int ket_calibrate_copy(uint64_t bytes, int reps, int nontemporal, double* gbps);
+
 /* ---- timing on the state's stream ---- */
 /* Record event slot (0..7) on the state's stream; elapsed ms between two. */
 int ket_sv_event_record(ket_sv* sv, int slot);

@@ -1,0 +1,115 @@
+"""Autotuner for the launch shape of the per-gate passes.
+
+Times a fixed probe circuit (dense, controlled, diagonal and permuting gates
+on low, middle and high qubits, so every access pattern of the plain form is
+represented) on a resident state for every point of the search space the C
+ABI exposes (ket_tuning: items per thread, nontemporal access, persistent
+grid), with per-launch HIP events, and keeps the fastest. Results are stored
+per device and precision in a JSON table that `apply_table` loads.
+
+    python -m ket_amd.autotune --qubits 30 --out table.json
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+
+from . import native as N
+from .gates import Circuit
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+
+SPACE = {"items_per_thread": [2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32]}
+
+
+def probe_circuit(n: int) -> Circuit:
+    c = Circuit(n)
+    lo, mid, hi = 0, n // 2, n - 1
+    for q in (lo, 2, mid, hi):
+        c.u(q, 0.3, 0.2, 0.1)          # dense 2x2
+    c.h(1)
+    c.cry(hi, mid, 0.4)                 # controlled dense
+    c.cry(lo, mid, 0.4)
+    for a, b in ((mid, hi), (lo, hi), (5, mid)):
+        c.cp(a, b, 0.7)                 # two-bit diagonal
+    c.t(mid)                            # one-bit diagonal
+    c.x(lo)                             # permutes
+    c.cx(mid, hi)
+    c.swap(3, hi)
+    c.ccx(lo, mid, hi - 1)
+    return c
+
+
+def measure(sv, circ, reps: int) -> dict:
+    sv.profile_circuit(circ, N.FUSE_NONE)  # warm
+    tot_ms, tot_bytes = 0.0, 0.0
+    by = {}
+    for _ in range(reps):
+        for r in sv.profile_circuit(circ, N.FUSE_NONE):
+            tot_ms += r["ms"]
+            tot_bytes += r["alg_bytes"]
+            d = by.setdefault(r["kind"], [0.0, 0.0])
+            d[0] += r["ms"]
+            d[1] += r["alg_bytes"]
+    return {"ms": tot_ms / reps, "GBps": tot_bytes / (tot_ms / 1e3) / 1e9,
+            "by_kind_GBps": {k: v[1] / (v[0] / 1e3) / 1e9 for k, v in by.items()}}
+
+
+def tune(n: int = 30, precision: int = N.C128, reps: int = 3, log=print) -> dict:
+    circ = probe_circuit(n)
+    results = []
+    with N.StateVector(n, precision) as sv:
+        for ipt, nt, bpc in itertools.product(*SPACE.values()):
+            N.set_tuning(ipt, nt, bpc)
+            m = measure(sv, circ, reps)
+            m.update({"items_per_thread": ipt, "nontemporal": nt, "blocks_per_cu": bpc})
+            results.append(m)
+            log(f"[tune] ipt={ipt} nt={nt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
+                + " ".join(f"{k}={v:.0f}" for k, v in m["by_kind_GBps"].items()))
+    best = min(results, key=lambda r: r["ms"])
+    N.set_tuning(best["items_per_thread"], best["nontemporal"], best["blocks_per_cu"])
+    return {"device": N.device_info()["name"], "precision": precision, "qubits": n,
+            "best": {k: best[k] for k in SPACE}, "best_GBps": best["GBps"], "all": results}
+
+
+def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
+    """Set the tuning recorded for this precision, if the table has one."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get("c128" if precision == N.C128 else "c64")
+    if entry:
+        b = entry["best"]
+        N.set_tuning(b["items_per_thread"], b["nontemporal"], b["blocks_per_cu"])
+        return b
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    for nt in (0, 1):
+        log(f"[tune] copy kernel nt={nt}: {N.calibrate_copy(4 << 30, 10, nt):.0f} GB/s")
+    table = {}
+    for prec, key in ((N.C128, "c128"), (N.C64, "c64")):
+        table[key] = tune(args.qubits, prec, args.reps, log)
+        log(f"[tune] {key} best {table[key]['best']} {table[key]['best_GBps']:.0f} GB/s")
+    out = json.dumps(table, indent=1)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(out)
+    else:
+        print(out)
+
+
+if __name__ == "__main__":
+    main()

@@ -693,6 +693,65 @@ int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity, double* max_abs_diff
   return KET_OK;
 }
 
+int ket_set_tuning(const ket_tuning* t) {
+  if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
+  if (t->items_per_thread != 2 && t->items_per_thread != 4 && t->items_per_thread != 8)
+    return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 2, 4 or 8");
+  if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
+    return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
+  ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu});
+  return KET_OK;
+}
+
+int ket_get_tuning(ket_tuning* t) {
+  if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
+  const ketk::Tuning k = ketk::get_tuning();
+  t->items_per_thread = k.items_per_thread;
+  t->nontemporal = k.nontemporal;
+  t->blocks_per_cu = k.blocks_per_cu;
+  t->pad = 0;
+  return KET_OK;
+}
+
+int ket_calibrate_copy(uint64_t bytes, int reps, int nontemporal, double* gbps) {
+  if (!gbps || reps < 1 || bytes < 4096 || (bytes % 16))
+    return fail(KET_ERR_ARG, "ket_amd: calibrate_copy needs bytes >= 4096 (multiple of 16), reps >= 1");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&](int code) {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return code;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&a, bytes)) != hipSuccess) return done(hip_fail(e, "hipMalloc"));
+  if ((e = hipMalloc(&b, bytes)) != hipSuccess) return done(hip_fail(e, "hipMalloc"));
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+    return done(hip_fail(e, "hipStreamCreate"));
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+    return done(hip_fail(e, "hipEventCreate"));
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int blocks = cus * 8;
+  if ((e = ketk::launch_copy(st, a, b, bytes, nontemporal, blocks)) != hipSuccess)
+    return done(hip_fail(e, "copy"));
+  (void)hipEventRecord(e0, st);
+  for (int r = 0; r < reps; ++r)
+    if ((e = ketk::launch_copy(st, a, b, bytes, nontemporal, blocks)) != hipSuccess)
+      return done(hip_fail(e, "copy"));
+  (void)hipEventRecord(e1, st);
+  if ((e = hipEventSynchronize(e1)) != hipSuccess) return done(hip_fail(e, "sync"));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * (double)bytes * reps / (ms / 1e3) / 1e9;
+  return done(KET_OK);
+}
+
 int ket_sv_event_record(ket_sv* sv, int slot) {
   if (int st = check_sv(sv)) return st;
   if (slot < 0 || slot >= 8) return fail(KET_ERR_ARG, "ket_amd: event slot must be 0..7");

@@ -21,6 +21,20 @@ struct PairSpec {
 
 enum PairOp { OP_DENSE = 0, OP_CH = 1, OP_SWAP = 2 };
 
+// Launch shape of the per-gate passes (the autotuner's search space):
+// items each thread keeps in flight (2, 4, 8), nontemporal loads/stores, and
+// a cap of blocks per CU for a persistent grid (0 = one block per
+// 256 * items items).
+struct Tuning {
+  int items_per_thread;
+  int nontemporal;
+  int blocks_per_cu;
+};
+void set_tuning(const Tuning& t);
+Tuning get_tuning();
+hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes,
+                       int nt, int blocks);
+
 // precision: 0 = complex128, 1 = complex64.
 hipError_t launch_fill_basis(hipStream_t st, int precision, void* s,
                              uint64_t dim, uint64_t index);

@@ -26,6 +26,37 @@ using ketd::insert_zeros;
 
 constexpr int kThreads = 256;
 
+// Streaming loads/stores: NT=1 marks them nontemporal (the state is touched
+// once per pass, so keeping it out of the caches leaves them to the rest).
+// The builtins take clang vector types, so the HIP vector structs are viewed
+// as those.
+typedef double nd2 __attribute__((ext_vector_type(2)));
+typedef float nf2 __attribute__((ext_vector_type(2)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
+template <class V> struct Native;
+template <> struct Native<double2> { using type = nd2; };
+template <> struct Native<float2> { using type = nf2; };
+template <> struct Native<float4> { using type = nf4; };
+
+template <int NT, class V>
+__device__ __forceinline__ V ld(const V* p) {
+  using N = typename Native<V>::type;
+  N v;
+  if constexpr (NT) v = __builtin_nontemporal_load(reinterpret_cast<const N*>(p));
+  else v = *reinterpret_cast<const N*>(p);
+  V r;
+  __builtin_memcpy(&r, &v, sizeof(V));
+  return r;
+}
+template <int NT, class V>
+__device__ __forceinline__ void st(V* p, V x) {
+  using N = typename Native<V>::type;
+  N v;
+  __builtin_memcpy(&v, &x, sizeof(V));
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<N*>(p));
+  else *reinterpret_cast<N*>(p) = v;
+}
+
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_fill_basis(cplx<T>* s,
                                                           uint64_t dim,
@@ -37,7 +68,7 @@ __global__ __launch_bounds__(kThreads) void k_fill_basis(cplx<T>* s,
   }
 }
 
-template <class T, int OP, int ITEMS>
+template <class T, int OP, int ITEMS, int NT>
 __global__ __launch_bounds__(kThreads) void k_pair(cplx<T>* __restrict__ s,
                                                     PairSpec sp, cplx<T> m00,
                                                     cplx<T> m01, cplx<T> m10,
@@ -54,8 +85,8 @@ __global__ __launch_bounds__(kThreads) void k_pair(cplx<T>* __restrict__ s,
                   ? (insert_zeros(i, sp.npos, sp.pos) | sp.set_mask)
                   : ~uint64_t{0};
       if (i < sp.n_items) {
-        a[j] = s[lo[j]];
-        b[j] = s[lo[j] ^ sp.partner];
+        a[j] = ld<NT>(s + lo[j]);
+        b[j] = ld<NT>(s + (lo[j] ^ sp.partner));
       }
     }
 #pragma unroll
@@ -73,13 +104,13 @@ __global__ __launch_bounds__(kThreads) void k_pair(cplx<T>* __restrict__ s,
         x = b[j];
         y = a[j];
       }
-      s[lo[j]] = x;
-      s[lo[j] ^ sp.partner] = y;
+      st<NT>(s + lo[j], x);
+      st<NT>(s + (lo[j] ^ sp.partner), y);
     }
   }
 }
 
-template <class T, int ITEMS>
+template <class T, int ITEMS, int NT>
 __global__ __launch_bounds__(kThreads) void k_phase(cplx<T>* __restrict__ s,
                                                      PairSpec sp, cplx<T> ph) {
   const uint64_t per_block = (uint64_t)kThreads * ITEMS;
@@ -93,12 +124,12 @@ __global__ __launch_bounds__(kThreads) void k_phase(cplx<T>* __restrict__ s,
       idx[j] = (i < sp.n_items)
                    ? (insert_zeros(i, sp.npos, sp.pos) | sp.set_mask)
                    : ~uint64_t{0};
-      if (i < sp.n_items) a[j] = s[idx[j]];
+      if (i < sp.n_items) a[j] = ld<NT>(s + idx[j]);
     }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if (idx[j] == ~uint64_t{0}) continue;
-      s[idx[j]] = cmul<T>(a[j], ph);
+      st<NT>(s + idx[j], cmul<T>(a[j], ph));
     }
   }
 }
@@ -153,15 +184,6 @@ __global__ __launch_bounds__(kThreads) void k_convert(
     d[i] = ketd::mk<D>((D)s[i].x, (D)s[i].y);
 }
 
-// Blocks for a grid-stride pass: enough to cover the items up to 2^20
-// blocks (2^28 work-items, far below the 2^32 limit of one dispatch).
-inline unsigned grid_for(uint64_t items, uint64_t per_block) {
-  uint64_t g = (items + per_block - 1) / per_block;
-  if (g > (uint64_t{1} << 20)) g = uint64_t{1} << 20;
-  if (g == 0) g = 1;
-  return (unsigned)g;
-}
-
 inline unsigned capped_grid(uint64_t items) {
   uint64_t g = (items + kThreads - 1) / kThreads;
   const uint64_t cap = 256 * 32;  // CUs x resident groups for grid-stride
@@ -178,26 +200,73 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
+Tuning g_tuning{4, 0, 0};
+
+inline unsigned tuned_grid(uint64_t items, int per_thread) {
+  const uint64_t per_block = (uint64_t)kThreads * per_thread;
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g_tuning.blocks_per_cu > 0) {
+    const uint64_t cap = (uint64_t)256 * g_tuning.blocks_per_cu;
+    if (g > cap) g = cap;
+  }
+  if (g > (uint64_t{1} << 20)) g = uint64_t{1} << 20;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+template <class T, int OP, int ITEMS, int NT>
+void pair_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
+  const unsigned grid = tuned_grid(sp.n_items, ITEMS);
+  k_pair<T, OP, ITEMS, NT><<<grid, kThreads, 0, st>>>(p, sp, to_c<T>(m, 0), to_c<T>(m, 1),
+                                                      to_c<T>(m, 2), to_c<T>(m, 3));
+}
+
+template <class T, int OP>
+void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
+  const bool nt = g_tuning.nontemporal != 0;
+  switch (g_tuning.items_per_thread) {
+    case 2: nt ? pair_go<T, OP, 2, 1>(st, p, sp, m) : pair_go<T, OP, 2, 0>(st, p, sp, m); break;
+    case 8: nt ? pair_go<T, OP, 8, 1>(st, p, sp, m) : pair_go<T, OP, 8, 0>(st, p, sp, m); break;
+    default: nt ? pair_go<T, OP, 4, 1>(st, p, sp, m) : pair_go<T, OP, 4, 0>(st, p, sp, m); break;
+  }
+}
+
 template <class T>
 hipError_t pair_impl(hipStream_t st, void* s, const PairSpec& sp, int op,
                      const double m[8]) {
-  constexpr int ITEMS = 4;
-  const unsigned grid = grid_for(sp.n_items, (uint64_t)kThreads * ITEMS);
   auto* p = static_cast<cplx<T>*>(s);
-  const cplx<T> a = to_c<T>(m, 0), b = to_c<T>(m, 1), c = to_c<T>(m, 2),
-                d = to_c<T>(m, 3);
   switch (op) {
-    case OP_DENSE:
-      k_pair<T, OP_DENSE, ITEMS><<<grid, kThreads, 0, st>>>(p, sp, a, b, c, d);
-      break;
-    case OP_CH:
-      k_pair<T, OP_CH, ITEMS><<<grid, kThreads, 0, st>>>(p, sp, a, b, c, d);
-      break;
-    default:
-      k_pair<T, OP_SWAP, ITEMS><<<grid, kThreads, 0, st>>>(p, sp, a, b, c, d);
-      break;
+    case OP_DENSE: pair_items<T, OP_DENSE>(st, p, sp, m); break;
+    case OP_CH: pair_items<T, OP_CH>(st, p, sp, m); break;
+    default: pair_items<T, OP_SWAP>(st, p, sp, m); break;
   }
   return hipGetLastError();
+}
+
+template <class T, int ITEMS, int NT>
+void phase_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, cplx<T> ph) {
+  k_phase<T, ITEMS, NT><<<tuned_grid(sp.n_items, ITEMS), kThreads, 0, st>>>(p, sp, ph);
+}
+
+template <class T>
+hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
+  auto* p = static_cast<cplx<T>*>(s);
+  const bool nt = g_tuning.nontemporal != 0;
+  switch (g_tuning.items_per_thread) {
+    case 2: nt ? phase_go<T, 2, 1>(st, p, sp, ph) : phase_go<T, 2, 0>(st, p, sp, ph); break;
+    case 8: nt ? phase_go<T, 8, 1>(st, p, sp, ph) : phase_go<T, 8, 0>(st, p, sp, ph); break;
+    default: nt ? phase_go<T, 4, 1>(st, p, sp, ph) : phase_go<T, 4, 0>(st, p, sp, ph); break;
+  }
+  return hipGetLastError();
+}
+
+// Bandwidth calibration: a plain float4 copy, grid-stride.
+template <int NT>
+__global__ __launch_bounds__(kThreads) void k_copy(const float4* __restrict__ a,
+                                                    float4* __restrict__ b, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    st<NT>(b + i, ld<NT>(a + i));
 }
 
 }  // namespace
@@ -221,14 +290,23 @@ hipError_t launch_pair(hipStream_t st, int precision, void* s,
 
 hipError_t launch_phase(hipStream_t st, int precision, void* s,
                         const PairSpec& sp, double re, double im) {
-  constexpr int ITEMS = 4;
-  const unsigned grid = grid_for(sp.n_items, (uint64_t)kThreads * ITEMS);
-  if (precision == 0)
-    k_phase<double, ITEMS><<<grid, kThreads, 0, st>>>(
-        static_cast<double2*>(s), sp, ketd::mk<double>(re, im));
+  return precision == 0
+             ? phase_impl<double>(st, s, sp, ketd::mk<double>(re, im))
+             : phase_impl<float>(st, s, sp, ketd::mk<float>((float)re, (float)im));
+}
+
+void set_tuning(const Tuning& t) { g_tuning = t; }
+Tuning get_tuning() { return g_tuning; }
+
+hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes, int nt,
+                       int blocks) {
+  const uint64_t n = bytes / sizeof(float4);
+  if (nt)
+    k_copy<1><<<blocks, kThreads, 0, st>>>(static_cast<const float4*>(a),
+                                           static_cast<float4*>(b), n);
   else
-    k_phase<float, ITEMS><<<grid, kThreads, 0, st>>>(
-        static_cast<float2*>(s), sp, ketd::mk<float>((float)re, (float)im));
+    k_copy<0><<<blocks, kThreads, 0, st>>>(static_cast<const float4*>(a),
+                                           static_cast<float4*>(b), n);
   return hipGetLastError();
 }
 

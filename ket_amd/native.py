@@ -81,6 +81,9 @@ def lib():
             "ket_sv_expval_pauli_sum": ([vp, sz, vp, vp, vp, vp, P(dbl)], i32),
             "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
             "ket_sv_profile_circuit": ([vp, vp, sz, i32, i32, vp, sz, P(sz)], i32),
+            "ket_set_tuning": ([vp], i32),
+            "ket_get_tuning": ([vp], i32),
+            "ket_calibrate_copy": ([u64, i32, i32, P(dbl)], i32),
             "ket_sv_event_record": ([vp, i32], i32),
             "ket_sv_event_elapsed": ([vp, i32, i32, P(ctypes.c_float)], i32),
         }
@@ -107,6 +110,29 @@ def device_info() -> dict:
     cu, hbm = ctypes.c_int(0), ctypes.c_uint64(0)
     check(lib().ket_device_info(name, 256, ctypes.byref(cu), ctypes.byref(hbm)))
     return {"name": name.value.decode(), "compute_units": cu.value, "hbm_bytes": hbm.value}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+def set_tuning(items_per_thread=4, nontemporal=0, blocks_per_cu=0):
+    t = Tuning(items_per_thread, nontemporal, blocks_per_cu, 0)
+    check(lib().ket_set_tuning(ctypes.byref(t)))
+
+
+def get_tuning() -> dict:
+    t = Tuning()
+    check(lib().ket_get_tuning(ctypes.byref(t)))
+    return {"items_per_thread": t.items_per_thread, "nontemporal": t.nontemporal,
+            "blocks_per_cu": t.blocks_per_cu}
+
+
+def calibrate_copy(nbytes: int = 4 << 30, reps: int = 10, nontemporal: int = 0) -> float:
+    v = ctypes.c_double(0)
+    check(lib().ket_calibrate_copy(int(nbytes), int(reps), int(nontemporal), ctypes.byref(v)))
+    return v.value
 
 
 def pauli_masks(pauli: str, n: int):
