@@ -200,7 +200,8 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
-Tuning g_tuning{4, 0, 0};
+// Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json).
+Tuning g_tuning{2, 0, 0};
 
 inline unsigned tuned_grid(uint64_t items, int per_thread) {
   const uint64_t per_block = (uint64_t)kThreads * per_thread;
