@@ -135,7 +135,7 @@ int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
       break;
     case ketplan::Kind::Block:
       e = ketk::launch_block(sv->stream, sv->precision, sv->data, sv->n, o.k,
-                             o.qubits, pool_dev + o.pool_offset);
+                             o.n_terms, pool_dev + o.pool_offset);
       break;
   }
   if (e != hipSuccess) return hip_fail(e, "kernel launch");

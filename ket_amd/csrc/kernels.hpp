@@ -50,10 +50,28 @@ hipError_t launch_matrix_exact(hipStream_t st, int precision, void* s,
 hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
                           int src_precision, const void* src, uint64_t count);
 
-// Dense fused block with fused multiply-adds; complex64 runs on MFMA.
-// mat_dev: device matrix in the precision of the state (row-major complex).
+// Pool record of a fused block (see sv_block.hip). Matrix bit b of an
+// element index is qubit qubits[b]; offsets[e] = that element's index bits.
+// Trailing phase terms are sorted by the block-local bit they depend on:
+// [term_start[0], term_start[1]) are group constants, [term_start[j+1],
+// term_start[j+2]) depend on local bit j. Term masks hold only out-of-block
+// bits; a term applies to a group when all of them are set.
+struct BlockRecord {
+  int32_t k;
+  int32_t n_terms;
+  int32_t term_start[9];
+  int32_t pad;
+  uint32_t terms_offset;  // bytes from the record start: masks, then angles
+  uint32_t mat_offset;    // bytes from the record start: the matrix
+  int32_t sorted[8];      // block qubits ascending
+  uint64_t offsets[64];
+};
+
+// Fused block with fused multiply-adds; complex64 k = 4, 5 runs on MFMA.
+// rec_dev: device copy of a BlockRecord with its terms and its matrix in the
+// state's precision; n_terms as recorded in it.
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
-                        const int* qubits, const void* mat_dev);
+                        int n_terms, const void* rec_dev);
 
 // ---- reductions; `work` is device scratch of at least reduce_work_bytes.
 size_t reduce_work_bytes(uint64_t dim);

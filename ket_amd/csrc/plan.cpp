@@ -188,27 +188,20 @@ Op op_for_gate(const ket_gate& g, int n, int precision) {
 
 Op op_for_matrix(Plan& plan, int n, int precision, int k, const int* qubits,
                  const double* mat, bool exact) {
-  (void)n;
+  if (!exact) return op_for_block(plan, n, precision, k, qubits, mat, {});
   Op o;
-  o.kind = exact ? Kind::MatrixExact : Kind::Block;
+  o.kind = Kind::MatrixExact;
   o.k = k;
   for (int i = 0; i < k; ++i) o.qubits[i] = qubits[i];
   const std::size_t D = std::size_t{1} << k;
-  // Pool record: the matrix (complex128 for the exact path and the
-  // complex128 block path, complex64 for the complex64 block path), then k
-  // sorted qubit positions and 2^k element offsets as uint64.
-  const bool f32 = !exact && precision == 1;
-  const std::size_t mat_bytes = D * D * (f32 ? 8 : 16);
+  // Pool record: the complex128 matrix, then k sorted qubit positions and
+  // 2^k element offsets as uint64.
+  const std::size_t mat_bytes = D * D * 16;
   const std::size_t rec = mat_bytes + (k + D) * sizeof(uint64_t);
   std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
   plan.pool.resize(off + rec);
   unsigned char* p = plan.pool.data() + off;
-  if (f32) {
-    float* fm = reinterpret_cast<float*>(p);
-    for (std::size_t i = 0; i < 2 * D * D; ++i) fm[i] = (float)mat[i];
-  } else {
-    std::memcpy(p, mat, mat_bytes);
-  }
+  std::memcpy(p, mat, mat_bytes);
   uint64_t* tab = reinterpret_cast<uint64_t*>(p + mat_bytes);
   std::vector<int> sorted(qubits, qubits + k);
   std::sort(sorted.begin(), sorted.end());
@@ -264,62 +257,204 @@ void fuser_flush(std::vector<const ket_gate*>& pend, std::vector<int>& bq,
 
 }  // namespace
 
+bool diag_terms(const ket_gate& g, std::vector<Term>* out) {
+  const double pi = 3.14159265358979323846;
+  const int* q = g.qubits;
+  switch (g.type) {
+    case KET_Z: out->push_back({bit(q[0]), pi}); return true;
+    case KET_S: out->push_back({bit(q[0]), pi / 2}); return true;
+    case KET_SDG: out->push_back({bit(q[0]), -pi / 2}); return true;
+    case KET_T: out->push_back({bit(q[0]), pi / 4}); return true;
+    case KET_TDG: out->push_back({bit(q[0]), -pi / 4}); return true;
+    case KET_RZ:  // diag(e^{-i t/2}, e^{i t/2})
+      out->push_back({0, -g.params[0] / 2});
+      out->push_back({bit(q[0]), g.params[0]});
+      return true;
+    case KET_CZ: out->push_back({bit(q[0]) | bit(q[1]), pi}); return true;
+    case KET_CP: out->push_back({bit(q[0]) | bit(q[1]), g.params[0]}); return true;
+    case KET_CRZ:
+      out->push_back({bit(q[0]), -g.params[0] / 2});
+      out->push_back({bit(q[0]) | bit(q[1]), g.params[0]});
+      return true;
+    default:
+      return false;
+  }
+}
+
+Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
+                const double* mat_in, const std::vector<Term>& terms_in) {
+  (void)n;
+  const std::size_t D = std::size_t{1} << k;
+  uint64_t Q = 0;
+  for (int b = 0; b < k; ++b) Q |= bit(qubits[b]);
+  auto local = [&](int qb) {
+    for (int b = 0; b < k; ++b)
+      if (qubits[b] == qb) return b;
+    return -1;
+  };
+  // Fold terms with every bit inside the block into the matrix rows; sort
+  // the rest by the block bit they depend on.
+  std::vector<cx> mat(D * D);
+  std::memcpy(mat.data(), mat_in, D * D * sizeof(cx));
+  std::vector<std::vector<Term>> byj(k + 1);
+  for (const Term& t : terms_in) {
+    const uint64_t in = t.mask & Q, out = t.mask & ~Q;
+    const int nin = __builtin_popcountll(in);
+    if (out == 0 && nin >= 1) {
+      uint64_t lm = 0;
+      for (int qb = 0; qb < 64; ++qb)
+        if (in & bit(qb)) lm |= bit(local(qb));
+      const kethost::CosSin ph = kethost::cos_sin(t.angle);
+      for (std::size_t r = 0; r < D; ++r)
+        if ((r & lm) == lm)
+          for (std::size_t c = 0; c < D; ++c)
+            mat[r * D + c] = kethost::mul(cx{ph.c, ph.s}, mat[r * D + c]);
+    } else if (nin == 0) {
+      byj[0].push_back({out, t.angle});
+    } else {  // nin == 1 (terms have at most two bits)
+      byj[1 + local(__builtin_ctzll(in))].push_back({out, t.angle});
+    }
+  }
+  ketk::BlockRecord h{};
+  h.k = k;
+  int nt = 0;
+  for (int j = 0; j <= k; ++j) {
+    h.term_start[j] = nt;
+    nt += (int)byj[j].size();
+  }
+  h.term_start[k + 1] = nt;
+  for (int j = k + 2; j < 9; ++j) h.term_start[j] = nt;
+  h.n_terms = nt;
+  std::vector<int> sorted(qubits, qubits + k);
+  std::sort(sorted.begin(), sorted.end());
+  for (int b = 0; b < k; ++b) h.sorted[b] = sorted[b];
+  for (std::size_t e = 0; e < D; ++e) {
+    uint64_t o = 0;
+    for (int b = 0; b < k; ++b)
+      if (e & (std::size_t{1} << b)) o |= bit(qubits[b]);
+    h.offsets[e] = o;
+  }
+  const std::size_t hdr = (sizeof(h) + 15) & ~std::size_t{15};
+  h.terms_offset = (uint32_t)hdr;
+  const std::size_t after_terms = hdr + (std::size_t)nt * 16;
+  h.mat_offset = (uint32_t)((after_terms + 15) & ~std::size_t{15});
+  const bool f32 = precision == 1;
+  const std::size_t mat_bytes = D * D * (f32 ? 8 : 16);
+  std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
+  plan.pool.resize(off + h.mat_offset + mat_bytes);
+  unsigned char* p = plan.pool.data() + off;
+  std::memcpy(p, &h, sizeof(h));
+  uint64_t* tm = reinterpret_cast<uint64_t*>(p + h.terms_offset);
+  double* ta = reinterpret_cast<double*>(tm + nt);
+  int i = 0;
+  for (int j = 0; j <= k; ++j)
+    for (const Term& t : byj[j]) {
+      tm[i] = t.mask;
+      ta[i] = t.angle;
+      ++i;
+    }
+  if (f32) {
+    float* fm = reinterpret_cast<float*>(p + h.mat_offset);
+    for (std::size_t e = 0; e < D * D; ++e) {
+      fm[2 * e] = (float)mat[e].re;
+      fm[2 * e + 1] = (float)mat[e].im;
+    }
+  } else {
+    std::memcpy(p + h.mat_offset, mat.data(), mat_bytes);
+  }
+  Op o;
+  o.kind = Kind::Block;
+  o.k = k;
+  for (int b = 0; b < k; ++b) o.qubits[b] = qubits[b];
+  o.pool_offset = off;
+  o.line_fraction = 1.0;
+  o.flops_per_amp = 8.0 * (double)D;
+  o.n_terms = nt;
+  return o;
+}
+
+namespace {
+BlockDecider g_decider = nullptr;
+constexpr int kMaxTerms = 256;
+}  // namespace
+
+bool default_decider(const BlockChoice& c) {
+  // One block pass moves one sweep; its gates' own passes move
+  // single_sweeps. Prefer the block when it moves fewer bytes.
+  return c.single_sweeps > 1.0;
+}
+
+void set_block_decider(BlockDecider d) { g_decider = d; }
+
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
-                 int n, int precision, int block_k, Plan* plan) {
+                 int n, int precision, int block_k, Plan* plan,
+                 BlockDecider decide) {
+  if (!decide) decide = g_decider ? g_decider : default_decider;
   const int L = line_bits(precision);
+  const uint64_t all = n >= 64 ? ~uint64_t{0} : (bit(n) - 1);
   std::vector<std::size_t> rem(run);
   std::vector<char> taken;
+  std::vector<Term> terms, gt;
   while (!rem.empty()) {
     taken.assign(rem.size(), 0);
-    uint64_t Q = 0, blocked = 0;
-    std::vector<std::size_t> members;
+    uint64_t Q = 0, blocked_any = 0, blocked_nd = 0, F = 0;
+    std::vector<std::size_t> members, trailing;
+    terms.clear();
     for (std::size_t i = 0; i < rem.size(); ++i) {
       const ket_gate& g = gates[rem[i]];
       uint64_t gm = 0;
       for (int j = 0; j < g.n_qubits; ++j) gm |= bit(g.qubits[j]);
-      if (gm & blocked) {
-        blocked |= gm;
-      } else if (members.empty() || __builtin_popcountll(Q | gm) <= block_k) {
+      gt.clear();
+      const bool diag = diag_terms(g, &gt);
+      bool take = false, trail = false;
+      if (diag) {
+        if (!(gm & blocked_nd)) {
+          if (members.empty() || __builtin_popcountll(Q | gm) <= block_k) take = true;
+          else if ((int)(terms.size() + gt.size()) <= kMaxTerms) trail = true;
+        }
+      } else if (!(gm & (blocked_any | F))) {
+        if (members.empty() || __builtin_popcountll(Q | gm) <= block_k) take = true;
+      }
+      if (take) {
         Q |= gm;
         members.push_back(rem[i]);
         taken[i] = 1;
+      } else if (trail) {
+        F |= gm;
+        trailing.push_back(rem[i]);
+        terms.insert(terms.end(), gt.begin(), gt.end());
+        taken[i] = 1;
       } else {
-        blocked |= gm;
+        blocked_any |= gm;
+        if (!diag) blocked_nd |= gm;
+        if (blocked_any == all) break;
       }
-      if (blocked == (n >= 64 ? ~uint64_t{0} : (bit(n) - 1))) break;
     }
     std::vector<std::size_t> rest;
     for (std::size_t i = 0; i < rem.size(); ++i)
       if (!taken[i]) rest.push_back(rem[i]);
     rem.swap(rest);
 
-    // Per-gate passes move sum(f) sweeps; a block moves 1.
-    std::vector<Op> singles;
+    // The block's gates in program order.
+    std::vector<std::size_t> order(members);
+    order.insert(order.end(), trailing.begin(), trailing.end());
+    std::sort(order.begin(), order.end());
     double sweeps = 0.0;
-    for (std::size_t gi : members) {
-      singles.push_back(op_for_gate(gates[gi], n, precision));
-      sweeps += singles.back().line_fraction;
-    }
+    for (std::size_t gi : order) sweeps += op_for_gate(gates[gi], n, precision).line_fraction;
     const int k = __builtin_popcountll(Q);
-    if (members.size() == 1 || sweeps <= 1.0 || k == 1) {
-      if (k == 1 && members.size() > 1) {
-        // One-qubit run: its 2x2 product in one dense pass.
-        std::vector<kethost::GateRef> refs;
-        for (std::size_t gi : members)
-          refs.push_back({gates[gi].type, {0, 0, 0}, gates[gi].params});
-        const std::vector<cx> mat = kethost::block_matrix(refs, 1);
-        plan->ops.push_back(pair_op(n, L, ketk::OP_DENSE, gates[members[0]].qubits[0],
-                                    {}, mat.data()));
-        plan->ops.back().n_gates = (int)members.size();
-        ++plan->blocks;
-      } else {
-        for (Op& o : singles) plan->ops.push_back(o);
-      }
+    int low = 0;
+    for (int qb = 0; qb < L; ++qb)
+      if (Q & bit(qb)) ++low;
+    BlockChoice ch{k, (int)members.size(), (int)trailing.size(), (int)terms.size(),
+                   Q ? __builtin_ctzll(Q) : 0, low, sweeps, precision, n};
+    const bool single_gate = order.size() == 1;
+    if (single_gate || !decide(ch)) {
+      for (std::size_t gi : order) plan->ops.push_back(op_for_gate(gates[gi], n, precision));
       continue;
     }
     std::vector<int> bq;
-    for (int q = 0; q < 64; ++q)
-      if (Q & bit(q)) bq.push_back(q);
+    for (int qb = 0; qb < 64; ++qb)
+      if (Q & bit(qb)) bq.push_back(qb);
     std::vector<kethost::GateRef> refs;
     for (std::size_t gi : members) {
       const ket_gate& g = gates[gi];
@@ -330,9 +465,12 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
       refs.push_back(r);
     }
     const std::vector<cx> mat = kethost::block_matrix(refs, k);
-    plan->ops.push_back(
-        op_for_matrix(*plan, n, precision, k, bq.data(), &mat[0].re, false));
-    plan->ops.back().n_gates = (int)members.size();
+    if (k == 1 && trailing.empty()) {
+      plan->ops.push_back(pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data()));
+    } else {
+      plan->ops.push_back(op_for_block(*plan, n, precision, k, bq.data(), &mat[0].re, terms));
+    }
+    plan->ops.back().n_gates = (int)order.size();
     ++plan->blocks;
   }
 }
@@ -382,7 +520,7 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
     std::vector<std::size_t> run;
     for (std::size_t i = 0; i <= n_gates; ++i) {
       if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
-        if (!run.empty()) pack_blocks(gates, run, n, precision, block_k, &plan);
+        if (!run.empty()) pack_blocks(gates, run, n, precision, block_k, &plan, nullptr);
         run.clear();
         continue;
       }

@@ -24,7 +24,19 @@ struct Op {
   double line_fraction = 1.0;               // f of the byte model
   double flops_per_amp = 0.0;
   int n_gates = 0;                          // unitary gates this pass covers
+  int n_terms = 0;                          // Block: trailing phase terms
 };
+
+// A diagonal phase term: multiply by exp(i * angle) where all bits of mask
+// are set (mask 0: always).
+struct Term {
+  uint64_t mask;
+  double angle;
+};
+
+// The phase terms of a diagonal gate (Z, S, Sdg, T, Tdg, Rz, CZ, CP, CRz);
+// false for any other gate.
+bool diag_terms(const ket_gate& g, std::vector<Term>* out);
 
 struct Plan {
   std::vector<Op> ops;
@@ -53,9 +65,32 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
                  int precision, int fusion, int block_k, Plan* out,
                  std::string* err);
 
-// The k-qubit packer (KET_FUSE_BLOCKS) over unitary gate indices [first,
-// last) of `gates`; `cut_cost` optional learned boundary scores.
+// A fused block pass: matrix bit b = qubits[b], trailing phase terms.
+Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
+                const double* mat /* 4^k complex128 */,
+                const std::vector<Term>& terms);
+
+// Decides whether a packed block is emitted as one block pass (true) or as
+// its gates' own passes; the default compares algorithmic bytes, the learned
+// planner substitutes its cost model.
+struct BlockChoice {
+  int k;               // block width
+  int n_members;       // gates in the dense matrix
+  int n_trailing;      // gates attached as phase terms
+  int n_terms;
+  int lowest_qubit;    // lowest block qubit
+  int low_qubits;      // block qubits below the line width
+  double single_sweeps;  // sum of the line fractions of the per-gate passes
+  int precision;
+  int n;
+};
+using BlockDecider = bool (*)(const BlockChoice&);
+bool default_decider(const BlockChoice& c);
+void set_block_decider(BlockDecider d);  // nullptr restores the default
+
+// The k-qubit packer (KET_FUSE_BLOCKS) over the unitary gate indices `run`.
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
-                 int n, int precision, int block_k, Plan* plan);
+                 int n, int precision, int block_k, Plan* plan,
+                 BlockDecider decide);
 
 }  // namespace ketplan

@@ -1,12 +1,23 @@
-// Fused dense k-qubit blocks: out = M * in on every group of 2^k amplitudes
-// that differ only in the block's qubits, with fused multiply-adds.
+// Fused dense k-qubit blocks with trailing diagonal phases.
 //
-// Pool record layout (written by the planner): the row-major 2^k x 2^k
-// matrix in the state's precision, then k sorted qubit positions and 2^k
-// element offsets (uint64).
+// One pass applies   out = Phase(x) * (M * in)   to every group of 2^k
+// amplitudes that differ only in the block's qubits Q: M is the block's
+// 2^k x 2^k matrix, Phase(x) = exp(i * (C(g) + sum_{j in Q, bit j of x set}
+// A_j(g))) collects diagonal gates (CZ, CP, Z, S, T, Rz, ...) that the
+// planner attached after the block although they touch qubits outside Q. For
+// a group g (fixed out-of-Q bits) C and A_j are sums of term angles whose
+// out-of-Q bits are all set in g, so they are evaluated once per group.
 //
-// This is the portable vector path: one thread per group, the matrix
-// broadcast from LDS, each output row accumulated with fma().
+// complex64, k in {4, 5}: MFMA. M is held in its real form
+//   [[Re M, -Im M], [Im M, Re M]]  (2D x 2D, D = 2^k)
+// as v_mfma_f32_16x16x4f32 A-operand registers for the whole kernel; each
+// wave streams tiles of 16 groups (B operand: real and imaginary parts of
+// the inputs, 4 per lane per k-step) and each lane finishes D/4 outputs.
+// Every other case: one thread per group, the matrix broadcast from LDS,
+// rows accumulated with fma().
+//
+// Pool record (written by the planner, 256-byte aligned):
+//   BlockRecord header | term masks (uint64) | term angles (double) | matrix
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -14,33 +25,88 @@ namespace ketk {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMaxTerms = 256;
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <class T, int K>
-__global__ __launch_bounds__(kThreads) void k_block_vec(
-    ketd::cplx<T>* __restrict__ s, uint64_t n_groups,
-    const ketd::cplx<T>* __restrict__ mat) {
+__device__ __forceinline__ uint64_t spread(uint64_t g, const int* sorted, int k) {
+  for (int b = 0; b < k; ++b) {
+    const int q = sorted[b];
+    g = ((g >> q) << (q + 1)) | (g & ((uint64_t{1} << q) - 1));
+  }
+  return g;
+}
+
+// Group constants of the trailing phase: C and A_j (j < k) for base x.
+template <int KMAX>
+__device__ __forceinline__ void group_angles(uint64_t x, const BlockRecord& h,
+                                             const uint64_t* tm, const double* th,
+                                             double& C, double (&A)[KMAX]) {
+  C = 0.0;
+  for (int t = h.term_start[0]; t < h.term_start[1]; ++t)
+    if ((x & tm[t]) == tm[t]) C += th[t];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    A[j] = 0.0;
+    if (j < h.k)
+      for (int t = h.term_start[j + 1]; t < h.term_start[j + 2]; ++t)
+        if ((x & tm[t]) == tm[t]) A[j] += th[t];
+  }
+}
+
+__device__ __forceinline__ double reduce_2pi(double a) {
+  const double two_pi = 6.283185307179586476925287;
+  return a - two_pi * rint(a / two_pi);
+}
+
+template <class T>
+__device__ __forceinline__ ketd::cplx<T> rotate(ketd::cplx<T> v, double angle);
+template <>
+__device__ __forceinline__ double2 rotate<double>(double2 v, double angle) {
+  double s, c;
+  sincos(reduce_2pi(angle), &s, &c);
+  return make_double2(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+template <>
+__device__ __forceinline__ float2 rotate<float>(float2 v, double angle) {
+  float s, c;
+  sincosf((float)reduce_2pi(angle), &s, &c);
+  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+}
+
+// ---- vector path (any k <= 6, both precisions) ----
+template <class T, int K, bool PH>
+__global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restrict__ s,
+                                                         uint64_t n_groups,
+                                                         const unsigned char* __restrict__ rec) {
   constexpr int D = 1 << K;
   __shared__ ketd::cplx<T> M[D * D];
   __shared__ uint64_t off[D];
-  const uint64_t* tab = reinterpret_cast<const uint64_t*>(mat + D * D);
+  __shared__ uint64_t tm[PH ? kMaxTerms : 1];
+  __shared__ double th[PH ? kMaxTerms : 1];
+  const BlockRecord& h = *reinterpret_cast<const BlockRecord*>(rec);
+  const auto* mat = reinterpret_cast<const ketd::cplx<T>*>(rec + h.mat_offset);
   for (int i = threadIdx.x; i < D * D; i += kThreads) M[i] = mat[i];
-  for (int i = threadIdx.x; i < D; i += kThreads) off[i] = tab[K + i];
+  for (int i = threadIdx.x; i < D; i += kThreads) off[i] = h.offsets[i];
+  if constexpr (PH) {
+    const auto* gm = reinterpret_cast<const uint64_t*>(rec + h.terms_offset);
+    const auto* ga = reinterpret_cast<const double*>(gm + h.n_terms);
+    for (int i = threadIdx.x; i < h.n_terms; i += kThreads) {
+      tm[i] = gm[i];
+      th[i] = ga[i];
+    }
+  }
   int sorted[K];
 #pragma unroll
-  for (int b = 0; b < K; ++b) sorted[b] = (int)tab[b];
+  for (int b = 0; b < K; ++b) sorted[b] = h.sorted[b];
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x; g < n_groups;
-       g += stride) {
-    uint64_t base = g;
-#pragma unroll
-    for (int b = 0; b < K; ++b) {
-      const int q = sorted[b];
-      base = ((base >> q) << (q + 1)) | (base & ((uint64_t{1} << q) - 1));
-    }
+  for (uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x; g < n_groups; g += stride) {
+    const uint64_t base = spread(g, sorted, K);
     ketd::cplx<T> in[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) in[i] = s[base | off[i]];
+    double C = 0.0, A[K];
+    if constexpr (PH) group_angles<K>(base, h, tm, th, C, A);
 #pragma unroll 1
     for (int r = 0; r < D; ++r) {
       T re = 0, im = 0;
@@ -52,44 +118,154 @@ __global__ __launch_bounds__(kThreads) void k_block_vec(
         im = fma(m.x, in[c].y, im);
         im = fma(m.y, in[c].x, im);
       }
-      s[base | off[r]] = ketd::mk<T>(re, im);
+      ketd::cplx<T> o = ketd::mk<T>(re, im);
+      if constexpr (PH) {
+        double ang = C;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          if ((r >> j) & 1) ang += A[j];
+        o = rotate<T>(o, ang);
+      }
+      s[base | off[r]] = o;
     }
   }
 }
 
-inline unsigned capped_grid(uint64_t items) {
+// ---- MFMA path (complex64, k = 4 or 5) ----
+template <int K, bool PH>
+__global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
+                                                          uint64_t n_groups,
+                                                          const unsigned char* __restrict__ rec) {
+  constexpr int D = 1 << K;
+  constexpr int RB = 2 * D / 16;  // 16-row blocks of the real form
+  constexpr int KS = 2 * D / 4;   // 4-deep k-steps
+  constexpr int NE = D / 4;       // inputs and outputs per lane per tile
+  __shared__ uint64_t off[D];
+  __shared__ uint64_t tm[PH ? kMaxTerms : 1];
+  __shared__ double th[PH ? kMaxTerms : 1];
+  const BlockRecord& h = *reinterpret_cast<const BlockRecord*>(rec);
+  for (int i = threadIdx.x; i < D; i += kThreads) off[i] = h.offsets[i];
+  if constexpr (PH) {
+    const auto* gm = reinterpret_cast<const uint64_t*>(rec + h.terms_offset);
+    const auto* ga = reinterpret_cast<const double*>(gm + h.n_terms);
+    for (int i = threadIdx.x; i < h.n_terms; i += kThreads) {
+      tm[i] = gm[i];
+      th[i] = ga[i];
+    }
+  }
+  int sorted[K];
+#pragma unroll
+  for (int b = 0; b < K; ++b) sorted[b] = h.sorted[b];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15;  // group within the tile (B/D column)
+  const int q = lane >> 4;    // k index within a k-step (A column / B row)
+  // A operand: lane holds A[row = rb*16 + col][kk = ks*4 + q] of the real form.
+  const auto* M = reinterpret_cast<const float2*>(rec + h.mat_offset);
+  float a[RB][KS];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int row = rb * 16 + col, c = ks * 4 + q;
+      const float2 m = M[(row % D) * D + (c % D)];
+      const bool im_r = row >= D, im_c = c >= D;
+      a[rb][ks] = !im_r ? (!im_c ? m.x : -m.y) : (!im_c ? m.y : m.x);
+    }
+
+  const uint64_t n_tiles = n_groups / 16;
+  const uint64_t wave = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const uint64_t n_waves = (uint64_t)gridDim.x * (kThreads / 64);
+  for (uint64_t tile = wave; tile < n_tiles; tile += n_waves) {
+    const uint64_t base = spread(tile * 16 + col, sorted, K);
+    float2 in[NE];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) in[i] = s[base | off[q + 4 * i]];
+    floatx4 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const float b = ks < NE ? in[ks].x : in[ks - NE].y;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[rb], 0, 0, 0);
+    }
+    double C = 0.0, A[K];
+    if constexpr (PH) group_angles<K>(base, h, tm, th, C, A);
+#pragma unroll
+    for (int rb = 0; rb < RB / 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int e = rb * 16 + 4 * q + v;
+        float2 o = make_float2(acc[rb][v], acc[rb + RB / 2][v]);
+        if constexpr (PH) {
+          double ang = C;
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            if ((e >> j) & 1) ang += A[j];
+          o = rotate<float>(o, ang);
+        }
+        s[base | off[e]] = o;
+      }
+  }
+}
+
+inline unsigned capped_grid(uint64_t items, uint64_t cap) {
   uint64_t g = (items + kThreads - 1) / kThreads;
-  const uint64_t cap = 256 * 16;
   if (g > cap) g = cap;
   if (g == 0) g = 1;
   return (unsigned)g;
 }
 
-template <class T>
-hipError_t block_vec(hipStream_t st, void* s, int n, int k, const void* mat) {
-  const uint64_t groups = (uint64_t{1} << n) >> k;
-  const unsigned g = capped_grid(groups);
+template <class T, bool PH>
+hipError_t block_vec(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
+  const unsigned g = capped_grid(groups, 256 * 16);
   auto* p = static_cast<ketd::cplx<T>*>(s);
-  auto* m = static_cast<const ketd::cplx<T>*>(mat);
   switch (k) {
-    case 1: k_block_vec<T, 1><<<g, kThreads, 0, st>>>(p, groups, m); break;
-    case 2: k_block_vec<T, 2><<<g, kThreads, 0, st>>>(p, groups, m); break;
-    case 3: k_block_vec<T, 3><<<g, kThreads, 0, st>>>(p, groups, m); break;
-    case 4: k_block_vec<T, 4><<<g, kThreads, 0, st>>>(p, groups, m); break;
-    case 5: k_block_vec<T, 5><<<g, kThreads, 0, st>>>(p, groups, m); break;
-    case 6: k_block_vec<T, 6><<<g, kThreads, 0, st>>>(p, groups, m); break;
+    case 1: k_block_vec<T, 1, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
+    case 2: k_block_vec<T, 2, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
+    case 3: k_block_vec<T, 3, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
+    case 4: k_block_vec<T, 4, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
+    case 5: k_block_vec<T, 5, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
+    case 6: k_block_vec<T, 6, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+template <bool PH>
+hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
+  // Waves stream tiles of 16 groups; a grid of 8 blocks (32 waves) per CU
+  // keeps the matrix registers loaded once per wave.
+  const uint64_t tiles = groups / 16;
+  uint64_t blocks = (tiles + 3) / 4;
+  if (blocks > 256 * 8) blocks = 256 * 8;
+  if (blocks == 0) blocks = 1;
+  auto* p = static_cast<float2*>(s);
+  if (k == 4)
+    k_block_mfma<4, PH><<<(unsigned)blocks, kThreads, 0, st>>>(p, groups, rec);
+  else
+    k_block_mfma<5, PH><<<(unsigned)blocks, kThreads, 0, st>>>(p, groups, rec);
   return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
-                        const int* qubits, const void* mat_dev) {
-  (void)qubits;
-  return precision == 0 ? block_vec<double>(st, s, n, k, mat_dev)
-                        : block_vec<float>(st, s, n, k, mat_dev);
+                        int n_terms, const void* rec_dev) {
+  if (k < 1 || k > 6 || k > n || n_terms < 0 || n_terms > kMaxTerms) return hipErrorInvalidValue;
+  const auto* rec = static_cast<const unsigned char*>(rec_dev);
+  const uint64_t groups = (uint64_t{1} << n) >> k;
+  const bool ph = n_terms > 0;
+  if (precision == 1 && (k == 4 || k == 5) && groups >= 16)
+    return ph ? block_mfma<true>(st, s, groups, k, rec) : block_mfma<false>(st, s, groups, k, rec);
+  if (precision == 0)
+    return ph ? block_vec<double, true>(st, s, groups, k, rec)
+              : block_vec<double, false>(st, s, groups, k, rec);
+  return ph ? block_vec<float, true>(st, s, groups, k, rec)
+            : block_vec<float, false>(st, s, groups, k, rec);
 }
 
 }  // namespace ketk

@@ -127,3 +127,29 @@ def random_mixed(n: int, depth: int, seed: int) -> Circuit:
             else:
                 getattr(c, name)(*qs)
     return c
+
+
+def diag_mix(n: int, depth: int, seed: int) -> Circuit:
+    """Dense gates interleaved with every diagonal gate type (Z, S, Sdg, T,
+    Tdg, Rz, CZ, CP, CRz) on random, often distant, qubit pairs."""
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for _ in range(depth):
+        for q in range(n):
+            c.u(q, *rng.uniform(0.0, TWO_PI, 3))
+        for _ in range(n):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            kind = int(rng.integers(9))
+            if kind == 0: c.cz(a, b)
+            elif kind == 1: c.cp(a, b, rng.uniform(0.0, TWO_PI))
+            elif kind == 2: c.crz(a, b, rng.uniform(0.0, TWO_PI))
+            elif kind == 3: c.rz(a, rng.uniform(0.0, TWO_PI))
+            elif kind == 4: c.z(a)
+            elif kind == 5: c.s(a)
+            elif kind == 6: c.sdg(a)
+            elif kind == 7: c.t(a)
+            else: c.tdg(a)
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.cx(a, b)
+        c.h(int(rng.integers(n)))
+    return c

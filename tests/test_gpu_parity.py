@@ -57,6 +57,8 @@ CIRCUITS = {
     "qft12": lambda: W.qft_u(12, seed=3),
     "hea10": lambda: W.hea(10, 8, seed=4),
     "bwm13": lambda: W.brickwork_mixed(13, 12, seed=5),
+    "diag12": lambda: W.diag_mix(12, 6, seed=6),
+    "qft14": lambda: W.qft_u(14, seed=8),
 }
 
 
