@@ -156,6 +156,39 @@ int ket_sv_expval_pauli_sum(ket_sv* sv, size_t n_terms, const uint64_t* flip,
 int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity,
                     double* max_abs_diff);
 
+/* ---- batched sweeps: many circuits of uneven widths (no counterpart in
+ *      the reference, which runs one circuit per call; its equivalent is a
+ *      loop of run() + expval(State, PauliSum), src/simulator.cpp:554-558,
+ *      src/observable.cpp:85-91) ---- */
+typedef struct ket_batch ket_batch;
+typedef struct ket_batch_stats {
+  uint64_t instances_lds;  /* instances held whole in LDS */
+  uint64_t instances_hbm;  /* instances in the HBM arena */
+  uint64_t launches;       /* kernel launches of the sweep */
+  uint64_t gates;          /* unitary gates over all instances */
+  double alg_bytes;        /* HBM bytes of the arena passes (line model) */
+  double plan_seconds;     /* host planning time */
+  double device_ms;        /* device time of the sweep (HIP events) */
+  int32_t graph_replayed;  /* 1 when a captured hipGraph was replayed */
+  int32_t pad;
+} ket_batch_stats;
+
+/* n_instances states, instance i of n_qubits[i] (1..30) qubits. */
+int ket_batch_create(size_t n_instances, const int* n_qubits, int precision,
+                     ket_batch** out);
+int ket_batch_destroy(ket_batch* b);
+/* Evolves every instance from |0...0> through its gates
+ * gates[gate_off[i] .. gate_off[i+1]) and writes
+ * out[i] = sum_t coeff[t] <Z-string zmask[t]> over its diagonal observable
+ * terms t in [term_off[i], term_off[i+1]) (the Z/I Pauli strings of
+ * ket's expval as sign masks). block_k: width of the fused blocks of
+ * instances that do not fit in LDS. use_graph: capture the launch sequence in
+ * a hipGraph and replay it while the sweep's structure is unchanged. */
+int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
+                  const uint64_t* zmask, const double* coeff,
+                  const uint64_t* term_off, int block_k, int use_graph,
+                  double* out, ket_batch_stats* stats);
+
 /* ---- per-launch profile (the cost model's training data) ---- */
 typedef enum ket_launch_kind {
   KET_LAUNCH_DENSE1Q = 0,  /* 2x2 on a target, optional controls */

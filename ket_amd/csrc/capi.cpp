@@ -18,26 +18,21 @@
 #include "kernels.hpp"
 #include "ket_amd.h"
 #include "plan.hpp"
+#include "status.hpp"
+
+namespace {
+thread_local std::string g_err;
+}  // namespace
+
+namespace ketabi {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace ketabi
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-int hip_fail(hipError_t e, const char* what) {
-  return fail(e == hipErrorOutOfMemory ? KET_ERR_ALLOC : KET_ERR_DEVICE,
-              std::string("ket_amd: ") + what + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(expr, what)                     \
-  do {                                          \
-    hipError_t e_ = (expr);                     \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
+using ketabi::fail;
+using ketabi::hip_fail;
+#define HIP_TRY(expr, what) KET_HIP_TRY(expr, what)
 
 constexpr int kMaxQubits = 40;
 

@@ -73,6 +73,42 @@ struct BlockRecord {
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
                         int n_terms, const void* rec_dev);
 
+// The same block pass on `instances` states of n qubits laid out back to
+// back (instance i at s + i * 2^n amplitudes), instance i's record at
+// rec_dev + i * rec_stride bytes (all records share k and n_terms).
+hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, int k,
+                                int n_terms, const void* rec_dev, uint64_t rec_stride,
+                                unsigned instances);
+
+// ---- batched small instances held entirely in LDS (sv_batch.hip) ----
+// One op of an instance's circuit in LDS: pair and phase passes over the
+// whole 2^n state, as the per-gate kernels define them.
+struct LdsOp {
+  int32_t kind;       // 0 dense, 1 CH, 2 permute, 3 phase
+  int32_t pad;
+  uint64_t fixed;     // bits fixed by the pass (target/controls or phase bits)
+  uint64_t set;       // their values for the visited index (`lo`)
+  uint64_t partner;   // lo ^ partner is the pair's other index
+  double m[8];        // 2x2 matrix, or the phase in m[0..1]
+};
+// Instance i (block i): n_qubits[i], ops [op_off[i], op_off[i+1]), diagonal
+// observable terms [term_off[i], term_off[i+1]) of (z mask, coeff); writes
+// sum_t coeff_t <prod Z over mask_t> to out[i]. States start at |0...0>.
+hipError_t launch_batch_lds(hipStream_t st, int precision, unsigned instances,
+                            int max_qubits, const int* n_qubits, const LdsOp* ops,
+                            const uint64_t* op_off, const uint64_t* zmask,
+                            const double* coeff, const uint64_t* term_off, double* out);
+// |0...0> for `instances` back-to-back states of n qubits.
+hipError_t launch_fill_batched(hipStream_t st, int precision, void* s, int n,
+                               unsigned instances);
+// Diagonal Pauli sums over back-to-back states (all with n_terms terms):
+// out[i] = sum_t coeff[i*n_terms+t] * <Z-string zmask[i*n_terms+t]>.
+// work: >= instances * 64 doubles.
+hipError_t launch_expval_diag_batched(hipStream_t st, int precision, const void* s, int n,
+                                      unsigned instances, int n_terms,
+                                      const uint64_t* zmask, const double* coeff,
+                                      double* work, double* out);
+
 // ---- reductions; `work` is device scratch of at least reduce_work_bytes.
 size_t reduce_work_bytes(uint64_t dim);
 hipError_t launch_norm(hipStream_t st, int precision, const void* s,

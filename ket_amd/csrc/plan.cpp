@@ -388,7 +388,7 @@ void set_block_decider(BlockDecider d) { g_decider = d; }
 
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 BlockDecider decide) {
+                 BlockDecider decide, bool force_blocks) {
   if (!decide) decide = g_decider ? g_decider : default_decider;
   const int L = line_bits(precision);
   const uint64_t all = n >= 64 ? ~uint64_t{0} : (bit(n) - 1);
@@ -448,7 +448,7 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
     BlockChoice ch{k, (int)members.size(), (int)trailing.size(), (int)terms.size(),
                    Q ? __builtin_ctzll(Q) : 0, low, sweeps, precision, n};
     const bool single_gate = order.size() == 1;
-    if (single_gate || !decide(ch)) {
+    if (!force_blocks && (single_gate || !decide(ch))) {
       for (std::size_t gi : order) plan->ops.push_back(op_for_gate(gates[gi], n, precision));
       continue;
     }
@@ -465,7 +465,7 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
       refs.push_back(r);
     }
     const std::vector<cx> mat = kethost::block_matrix(refs, k);
-    if (k == 1 && trailing.empty()) {
+    if (k == 1 && trailing.empty() && !force_blocks) {
       plan->ops.push_back(pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data()));
     } else {
       plan->ops.push_back(op_for_block(*plan, n, precision, k, bq.data(), &mat[0].re, terms));

@@ -89,8 +89,10 @@ bool default_decider(const BlockChoice& c);
 void set_block_decider(BlockDecider d);  // nullptr restores the default
 
 // The k-qubit packer (KET_FUSE_BLOCKS) over the unitary gate indices `run`.
+// force_blocks: every packed group becomes a Block pass (one kernel kind
+// for the whole plan, as the batched engine needs).
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 BlockDecider decide);
+                 BlockDecider decide, bool force_blocks = false);
 
 }  // namespace ketplan

@@ -74,10 +74,13 @@ __device__ __forceinline__ float2 rotate<float>(float2 v, double angle) {
 }
 
 // ---- vector path (any k <= 6, both precisions) ----
+// One thread per group; `block` and `nblocks` enumerate the workgroups that
+// share this state (grid-stride over its groups).
 template <class T, int K, bool PH>
-__global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restrict__ s,
-                                                         uint64_t n_groups,
-                                                         const unsigned char* __restrict__ rec) {
+__device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
+                                               uint64_t n_groups,
+                                               const unsigned char* __restrict__ rec,
+                                               uint64_t block, uint64_t nblocks) {
   constexpr int D = 1 << K;
   __shared__ ketd::cplx<T> M[D * D];
   __shared__ uint64_t off[D];
@@ -99,8 +102,8 @@ __global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restric
 #pragma unroll
   for (int b = 0; b < K; ++b) sorted[b] = h.sorted[b];
   __syncthreads();
-  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x; g < n_groups; g += stride) {
+  const uint64_t stride = nblocks * kThreads;
+  for (uint64_t g = block * kThreads + threadIdx.x; g < n_groups; g += stride) {
     const uint64_t base = spread(g, sorted, K);
     ketd::cplx<T> in[D];
 #pragma unroll
@@ -129,6 +132,23 @@ __global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restric
       s[base | off[r]] = o;
     }
   }
+}
+
+template <class T, int K, bool PH>
+__global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restrict__ s,
+                                                         uint64_t n_groups,
+                                                         const unsigned char* __restrict__ rec) {
+  block_vec_body<T, K, PH>(s, n_groups, rec, blockIdx.x, gridDim.x);
+}
+
+// Batched: blockIdx.y = instance; instance i's state starts at
+// s + i * state_stride and its record at rec + i * rec_stride.
+template <class T, int K, bool PH>
+__global__ __launch_bounds__(kThreads) void k_block_vec_batched(
+    ketd::cplx<T>* __restrict__ s, uint64_t state_stride, uint64_t n_groups,
+    const unsigned char* __restrict__ rec, uint64_t rec_stride) {
+  block_vec_body<T, K, PH>(s + blockIdx.y * state_stride, n_groups,
+                           rec + blockIdx.y * rec_stride, blockIdx.x, gridDim.x);
 }
 
 // ---- MFMA path (complex64, k = 4 or 5) ----
@@ -251,7 +271,43 @@ hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const uns
   return hipGetLastError();
 }
 
+template <class T, bool PH>
+hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uint64_t groups,
+                             int k, const unsigned char* rec, uint64_t rec_stride,
+                             unsigned instances) {
+  uint64_t gx = (groups + kThreads - 1) / kThreads;
+  if (gx > 64) gx = 64;
+  if (gx == 0) gx = 1;
+  const dim3 grid((unsigned)gx, instances);
+  auto* p = static_cast<ketd::cplx<T>*>(s);
+  switch (k) {
+    case 1: k_block_vec_batched<T, 1, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    case 2: k_block_vec_batched<T, 2, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    case 3: k_block_vec_batched<T, 3, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    case 4: k_block_vec_batched<T, 4, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    case 5: k_block_vec_batched<T, 5, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    case 6: k_block_vec_batched<T, 6, PH><<<grid, kThreads, 0, st>>>(p, state_stride, groups, rec, rec_stride); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, int k,
+                                int n_terms, const void* rec_dev, uint64_t rec_stride,
+                                unsigned instances) {
+  if (k < 1 || k > 6 || k > n || n_terms < 0 || n_terms > kMaxTerms || instances == 0 ||
+      instances > 65535)
+    return hipErrorInvalidValue;
+  const auto* rec = static_cast<const unsigned char*>(rec_dev);
+  const uint64_t dim = uint64_t{1} << n, groups = dim >> k;
+  if (precision == 0)
+    return n_terms ? block_vec_batched<double, true>(st, s, dim, groups, k, rec, rec_stride, instances)
+                   : block_vec_batched<double, false>(st, s, dim, groups, k, rec, rec_stride, instances);
+  return n_terms ? block_vec_batched<float, true>(st, s, dim, groups, k, rec, rec_stride, instances)
+                 : block_vec_batched<float, false>(st, s, dim, groups, k, rec, rec_stride, instances);
+}
 
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
                         int n_terms, const void* rec_dev) {

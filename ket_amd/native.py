@@ -33,6 +33,17 @@ class LaunchRecord(ctypes.Structure):
 LAUNCH_KINDS = ["dense1q", "ch", "permute", "phase", "matrix", "block"]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("instances_lds", ctypes.c_uint64), ("instances_hbm", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("gates", ctypes.c_uint64),
+                ("alg_bytes", ctypes.c_double), ("plan_seconds", ctypes.c_double),
+                ("device_ms", ctypes.c_double), ("graph_replayed", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("blocks", ctypes.c_uint64), ("alg_bytes", ctypes.c_double),
@@ -84,6 +95,9 @@ def lib():
             "ket_set_tuning": ([vp], i32),
             "ket_get_tuning": ([vp], i32),
             "ket_calibrate_copy": ([u64, i32, i32, P(dbl)], i32),
+            "ket_batch_create": ([sz, vp, i32, P(vp)], i32),
+            "ket_batch_destroy": ([vp], i32),
+            "ket_batch_run": ([vp, vp, vp, vp, vp, vp, i32, i32, vp, P(BatchStats)], i32),
             "ket_sv_event_record": ([vp, i32], i32),
             "ket_sv_event_elapsed": ([vp, i32, i32, P(ctypes.c_float)], i32),
         }
@@ -309,3 +323,58 @@ class StateVector:
         ms = ctypes.c_float(0)
         check(lib().ket_sv_event_elapsed(self._h, a, b, ctypes.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Many circuits of uneven widths evaluated in one sweep (ket_batch_*)."""
+
+    def __init__(self, n_qubits, precision: int = C128):
+        nq = np.ascontiguousarray(n_qubits, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().ket_batch_create(nq.size, _p(nq), int(precision), ctypes.byref(h)))
+        self._h = h
+        self.n_qubits = nq
+        self.precision = int(precision)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ket_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, circuits, observables, block_k: int = 4, use_graph: bool = True):
+        """circuits[i]: ket_amd.gates.Circuit; observables[i]: list of
+        (coeff, pauli) with only I/Z characters. Returns (values, stats)."""
+        recs = [c.records() for c in circuits]
+        goff = np.zeros(len(recs) + 1, np.uint64)
+        goff[1:] = np.cumsum([len(r) for r in recs])
+        gates = np.concatenate(recs) if recs else np.zeros(0, GATE_DTYPE)
+        zm, cf, toff = [], [], [0]
+        for c, obs in zip(circuits, observables):
+            for coeff, pauli in obs:
+                flip, sign, _ = pauli_masks(pauli, c.n_qubits)
+                if flip:
+                    raise ValueError("batched observables must be diagonal (I/Z only)")
+                zm.append(sign)
+                cf.append(coeff)
+            toff.append(len(zm))
+        zm = np.ascontiguousarray(zm, np.uint64)
+        cf = np.ascontiguousarray(cf, np.float64)
+        toff = np.ascontiguousarray(toff, np.uint64)
+        out = np.zeros(len(recs), np.float64)
+        st = BatchStats()
+        check(lib().ket_batch_run(self._h, _p(gates), _p(goff), _p(zm), _p(cf), _p(toff),
+                                  int(block_k), 1 if use_graph else 0, _p(out),
+                                  ctypes.byref(st)))
+        return out, st.as_dict()
