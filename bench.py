@@ -43,6 +43,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def hea_batch(count=4096, lo=10, hi=22, seed=0, angle_seed=0):
+    """Config 4: `count` hardware-efficient-ansatz instances, widths uniform
+    in [lo, hi], 8 layers each; observable sum_i Z_i Z_{i+1} (open chain)."""
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.integers(lo, hi + 1, count)]
+    circs = [W.hea(n, 8, seed=angle_seed * 1_000_003 + i) for i, n in enumerate(ns)]
+    obs = [[(1.0, p) for p in W.zz_chain(n)] for n in ns]
+    return ns, circs, obs
+
+
 def workload(name):
     if name == "qft30":
         return W.qft_u(30, seed=0), "qft30_u: 30-qubit QFT after one U layer (510 gates)"
@@ -107,12 +117,149 @@ def cpu_baseline(circ, threads, budget_s=20.0):
             "seconds": [t_s, t_l]}
 
 
+def emit(out):
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_brick33(args, circ, desc):
+    """Config 3 at its real size: fused complex64 (k = 5, 64 GiB) against the
+    plain complex128 form (128 GiB), both resident; 1-F between them."""
+    n, gates = circ.n_qubits, circ.n_unitary()
+    out = {"metric": METRIC}
+    fused = N.StateVector(n, N.C64)
+    log(f"[bench] brick33 fused complex64 k=5: {gates} gates")
+    w, e, st = time_steps(fused, circ, N.FUSE_BLOCKS, 5, args.steps, args.warmup)
+    ps_f = w / args.steps
+    log(f"[bench] fused: {ps_f:.2f} s/step, {gates / ps_f:.1f} gates/s, {st['launches']} launches, "
+        f"{st['alg_bytes'] / ps_f / 1e9:.0f} GB/s")
+    plain = N.StateVector(n, N.C128)
+    log("[bench] plain complex128 (one step)")
+    wp, ep, stp = time_steps(plain, circ, N.FUSE_NONE, 5, 1, 0)
+    f, dmax = fused.fidelity(plain)
+    log(f"[bench] plain: {wp:.2f} s, {gates / wp:.1f} gates/s, {stp['alg_bytes'] / wp / 1e9:.0f} GB/s "
+        f"({stp['alg_bytes'] / wp / 1e9 / HBM_PEAK_GBPS:.1%}); 1-F={1 - f:.2e} max|d|={dmax:.2e}")
+    out.update({
+        "value": gates / wp, "unit": "gates/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wp * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {"workload": desc, "n_qubits": n, "gates": gates, "form": "plain per-gate complex128"},
+        "hbm_GBps_algorithmic": stp["alg_bytes"] / wp / 1e9,
+        "roofline_frac_plain": stp["alg_bytes"] / wp / 1e9 / HBM_PEAK_GBPS,
+        "fused": {"c64_k5": {"s_per_step": ps_f, "gates_per_s": gates / ps_f,
+                             "alg_GBps": st["alg_bytes"] / ps_f / 1e9, "launches": st["launches"],
+                             "blocks": st["blocks"], "one_minus_F": 1 - f, "max_abs_err": dmax,
+                             "speedup_vs_plain": wp / ps_f}},
+        "device": N.device_info()["name"]})
+    plain.close()
+    fused.close()
+    return emit(out)
+
+
+def bench_hea(args):
+    """Config 4: 4096 instances, widths U{10..22}, one sweep per step with a
+    new parameter binding (two angle sets alternate), complex128 and complex64,
+    hipGraph replay; CPU baseline: ket::run + expval per instance."""
+    sets = [hea_batch(angle_seed=a) for a in (0, 1)]
+    ns = sets[0][0]
+    gates = sum(c.n_unitary() for c in sets[0][1])
+    log(f"[bench] hea4096: {len(ns)} instances, {gates} gates per sweep")
+    out = {"metric": METRIC}
+    res = {}
+    for label, prec in (("c128", N.C128), ("c64", N.C64)):
+        with N.Batch(ns, prec) as b:
+            for w in range(max(args.warmup, 1)):
+                b.run(sets[w % 2][1], sets[w % 2][2], block_k=4, use_graph=True)
+            t0 = time.perf_counter()
+            dev_ms, plan_s = 0.0, 0.0
+            for k in range(args.steps):
+                vals, st = b.run(sets[k % 2][1], sets[k % 2][2], block_k=4, use_graph=True)
+                dev_ms += st["device_ms"]
+                plan_s += st["plan_seconds"]
+            wall = (time.perf_counter() - t0) / args.steps
+            res[label] = {"s_per_sweep": wall, "instances_per_s": len(ns) / wall,
+                          "gates_per_s": gates / wall, "device_ms_per_sweep": dev_ms / args.steps,
+                          "plan_s_per_sweep": plan_s / args.steps, "launches": st["launches"],
+                          "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"]}
+            log(f"[bench] {label}: {wall:.3f} s/sweep ({dev_ms / args.steps:.1f} ms device, "
+                f"{plan_s / args.steps:.3f} s planning), {len(ns) / wall:.0f} instances/s, "
+                f"{st['launches']} launches")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        if O.ref_available():
+            rng = np.random.default_rng(7)
+            pick = rng.choice(len(ns), 32, replace=False)
+            t0 = time.perf_counter()
+            for i in pick:
+                O.ref_expval(sets[0][1][i], [p for _, p in sets[0][2][i]])
+            dt = time.perf_counter() - t0
+            frac_gates = sum(sets[0][1][i].n_unitary() for i in pick) / gates
+            cpu = {"value": len(pick) / dt, "unit": "instances/s", "cores": 1, "kind": "reference",
+                   "sample": f"ket::run + expval(PauliSum) on 32 random instances of the sweep "
+                             f"({frac_gates:.2%} of its gates), one process per instance, 1 thread"}
+            log(f"[bench] cpu: {cpu['value']:.1f} instances/s")
+    out.update({
+        "value": res["c128"]["instances_per_s"], "unit": "instances/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["c128"]["s_per_sweep"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": "hea4096: 4096 HEA instances (8 layers Ry,Rz + CZ ring), "
+                               "n ~ U{10..22}, sum Z_i Z_i+1", "gates_per_sweep": gates},
+        "forms": res, "cpu_baseline": cpu, "device": N.device_info()["name"]})
+    return emit(out)
+
+
+def bench_brick20(args):
+    """Config 1: 20-qubit brickwork, 20 layers (590 gates); final state via
+    ket's Fuser rule in complex128 (exact), 10^4 shots with mt19937{1}."""
+    from oracle import oracle as O
+    circ = W.brickwork(20, 20, seed=0)
+    gates = circ.n_unitary()
+    sv = N.StateVector(20, N.C128)
+    w, e, st = time_steps(sv, circ, N.FUSE_REFERENCE, 5, args.steps, args.warmup)
+    u = O.uniforms(1, 10_000)
+    sv.set_basis(0)
+    sv.apply_circuit(circ, N.FUSE_REFERENCE)
+    sv.sync()
+    t0 = time.perf_counter()
+    shots = sv.sample(u)
+    t_draw = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    shots2 = sv.sample(u)  # sampler tables reused
+    t_draw2 = time.perf_counter() - t0
+    run_s = w / args.steps
+    log(f"[bench] brick20: run {run_s * 1e3:.2f} ms ({gates / run_s:.0f} gates/s); 10^4 shots "
+        f"{t_draw * 1e3:.2f} ms (tables built) / {t_draw2 * 1e3:.2f} ms (reused)")
+    cpu = None
+    if not args.no_cpu and O.ref_available():
+        t = O.ref_time(circ, 1, 3)["seconds"]
+        t0 = time.perf_counter()
+        O.ref_sample(circ, 1, 10_000)
+        t_s = time.perf_counter() - t0
+        cpu = {"value": gates / min(t), "unit": "gates/s", "cores": 1, "kind": "reference",
+               "sample": f"ket::run of the full circuit (1 thread, best of 3: {min(t) * 1e3:.1f} ms); "
+                         f"simulate + 10^4 Simulation::sample: {t_s:.2f} s incl. process start"}
+    out = {"metric": METRIC, "value": gates / run_s, "unit": "gates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": run_s * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic",
+           "config": {"workload": "brick20: 20-qubit brickwork, 20 layers, 590 gates; 10^4 shots",
+                      "n_qubits": 20, "gates": gates},
+           "shots_10k_ms": t_draw * 1e3, "shots_10k_ms_reused": t_draw2 * 1e3,
+           "shots_reproducible": bool(np.array_equal(shots, shots2)),
+           "cpu_baseline": cpu, "device": N.device_info()["name"]}
+    sv.close()
+    return emit(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="qft30")
+    ap.add_argument("--workload", default="qft30",
+                    help="qft30 (default), brick33, hea4096, brick20, qftNN")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -123,6 +270,10 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.workload == "hea4096":
+        return bench_hea(args)
+    if args.workload == "brick20":
+        return bench_brick20(args)
     circ, desc = workload(args.workload)
     n = circ.n_qubits
     gates = circ.n_unitary()
@@ -131,6 +282,8 @@ def main():
         f"HBM={info['hbm_bytes'] / 2**30:.0f} GiB; workload {desc}")
 
     out = {"metric": METRIC}
+    if args.workload == "brick33":
+        return bench_brick33(args, circ, desc)
     plain = N.StateVector(n, N.C128)
     log(f"[bench] plain complex128: warmup {args.warmup}, steps {args.steps}")
     wall, ev_s, stats = time_steps(plain, circ, N.FUSE_NONE, 5, args.steps, args.warmup)

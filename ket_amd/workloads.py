@@ -84,7 +84,8 @@ def hea(n: int, layers: int = 8, seed: int = 0) -> Circuit:
             c.ry(q, rng.uniform(-math.pi, math.pi))
             c.rz(q, rng.uniform(-math.pi, math.pi))
         for q in range(n):
-            c.cz(q, (q + 1) % n)
+            if (q + 1) % n != q and not (n == 2 and q == 1):
+                c.cz(q, (q + 1) % n)
     return c
 
 
