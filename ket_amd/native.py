@@ -353,9 +353,10 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def run(self, circuits, observables, block_k: int = 4, use_graph: bool = True):
-        """circuits[i]: ket_amd.gates.Circuit; observables[i]: list of
-        (coeff, pauli) with only I/Z characters. Returns (values, stats)."""
+    @staticmethod
+    def pack(circuits, observables):
+        """Flattens circuits and diagonal observables into the arrays
+        ket_batch_run takes: (gates, gate_off, zmask, coeff, term_off)."""
         recs = [c.records() for c in circuits]
         goff = np.zeros(len(recs) + 1, np.uint64)
         goff[1:] = np.cumsum([len(r) for r in recs])
@@ -372,7 +373,16 @@ class Batch:
         zm = np.ascontiguousarray(zm, np.uint64)
         cf = np.ascontiguousarray(cf, np.float64)
         toff = np.ascontiguousarray(toff, np.uint64)
-        out = np.zeros(len(recs), np.float64)
+        return gates, goff, zm, cf, toff
+
+    def run(self, circuits, observables, block_k: int = 4, use_graph: bool = True):
+        """circuits[i]: ket_amd.gates.Circuit; observables[i]: list of
+        (coeff, pauli) with only I/Z characters. Returns (values, stats)."""
+        return self.run_packed(self.pack(circuits, observables), block_k, use_graph)
+
+    def run_packed(self, packed, block_k: int = 4, use_graph: bool = True):
+        gates, goff, zm, cf, toff = packed
+        out = np.zeros(len(goff) - 1, np.float64)
         st = BatchStats()
         check(lib().ket_batch_run(self._h, _p(gates), _p(goff), _p(zm), _p(cf), _p(toff),
                                   int(block_k), 1 if use_graph else 0, _p(out),
