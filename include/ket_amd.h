@@ -68,8 +68,11 @@ typedef enum ket_fusion {
                              (src/simulator.cpp:400-477); with KET_C128 the
                              result equals ket::run's value for value */
   KET_FUSE_BLOCKS = 2,    /* k-qubit packer: dense 2^k blocks (k = block_k) */
-  KET_FUSE_LEARNED = 3    /* packer whose block boundaries a trained cost
-                             model picks; falls back to KET_FUSE_BLOCKS */
+  KET_FUSE_LEARNED = 3,   /* packer whose block width and block-or-not
+                             choices a trained cost model picks; without a
+                             model it is KET_FUSE_BLOCKS */
+  KET_FUSE_BLOCKS_ALL = 4 /* the packer with every group emitted as a block
+                             (cost-model training data, batched sweeps) */
 } ket_fusion;
 
 typedef struct ket_sv ket_sv; /* a state vector resident on one GPU */
@@ -199,15 +202,22 @@ typedef enum ket_launch_kind {
   KET_LAUNCH_BLOCK = 5     /* fused dense 2^k block */
 } ket_launch_kind;
 
+#define KET_MLP_INPUTS 16
+#define KET_MLP_HIDDEN 64
+#define KET_MLP_PARAMS                                                  \
+  (KET_MLP_HIDDEN * KET_MLP_INPUTS + KET_MLP_HIDDEN +                   \
+   KET_MLP_HIDDEN * KET_MLP_HIDDEN + KET_MLP_HIDDEN + KET_MLP_HIDDEN + 1)
+
 typedef struct ket_launch_record {
   int32_t kind;          /* ket_launch_kind */
   int32_t k;             /* block width (0 for per-gate passes) */
   int32_t n_gates;       /* unitary gates the pass applies */
-  int32_t pad;
+  int32_t n_terms;       /* trailing phase terms of a block */
   uint64_t qubit_mask;   /* qubits the pass acts on */
   double alg_bytes;      /* algorithmic HBM bytes (128-byte line model) */
   double flops;
   double ms;             /* device time between events around the launch */
+  float features[KET_MLP_INPUTS]; /* the cost model's description of it */
 } ket_launch_record;
 
 /* Plans and runs the circuit like ket_sv_apply_circuit, timing every launch
@@ -216,6 +226,30 @@ typedef struct ket_launch_record {
 int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
                            int fusion, int block_k, ket_launch_record* out,
                            size_t cap, size_t* n_launches);
+
+/* ---- the learned planner's cost model (no counterpart in the reference,
+ *      whose Fuser is a fixed greedy rule, src/simulator.cpp:400-477) ----
+ * A 3-layer MLP, KET_MLP_INPUTS -> KET_MLP_HIDDEN -> KET_MLP_HIDDEN -> 1 with
+ * ReLU, predicting log(milliseconds) of one pass from the features in
+ * ket_launch_record. Parameters, row-major: W1[H][D], b1[H], W2[H][H],
+ * b2[H], W3[H], b3. Trained on the GPU (forward, backward, Adam kernels). */
+typedef struct ket_mlp ket_mlp;
+int ket_mlp_create(uint64_t seed, ket_mlp** out);
+int ket_mlp_destroy(ket_mlp* m);
+int ket_mlp_set_params(ket_mlp* m, const float* params, size_t n);
+int ket_mlp_get_params(ket_mlp* m, float* params, size_t n);
+/* y[i] = model(x[i*D .. i*D+D)) for n samples. */
+int ket_mlp_forward(ket_mlp* m, const float* x, size_t n, float* y);
+/* Gradient of L = mean (model(x_i) - t_i)^2 w.r.t. the parameters (grad gets
+ * KET_MLP_PARAMS floats; either output may be NULL). */
+int ket_mlp_gradients(ket_mlp* m, const float* x, const float* t, size_t n,
+                      float* grad, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999) on the batch; *loss before it. */
+int ket_mlp_train_step(ket_mlp* m, const float* x, const float* t, size_t n,
+                       float lr, float* loss);
+/* KET_FUSE_LEARNED plans with this model (NULL: greedy packing). The model
+ * must outlive its use; destroying it unsets it. */
+int ket_mlp_use_for_planning(ket_mlp* m);
 
 /* ---- launch shape of the per-gate passes (the autotuner's search space,
  *      process-wide; no counterpart in the reference, whose CPU loops have

@@ -115,7 +115,7 @@ int plan_blocks_only(const ket_gate* g, std::size_t ng, int n, int precision, in
       continue;
     }
     if (!run.empty())
-      ketplan::pack_blocks(g, run, n, precision, std::min(block_k, n), plan, nullptr, true);
+      ketplan::pack_blocks(g, run, n, precision, std::min(block_k, n), plan, {}, true);
     run.clear();
   }
   return KET_OK;

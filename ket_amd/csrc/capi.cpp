@@ -540,6 +540,8 @@ int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, in
         break;
     }
     r.n_gates = o.n_gates;
+    r.n_terms = o.n_terms;
+    ketplan::op_features(o, sv->n, sv->precision, r.features);
     r.alg_bytes = sweep * o.line_fraction;
     double touched = (double)sv->dim;
     if (o.kind == ketplan::Kind::Phase) touched = (double)o.spec.n_items;

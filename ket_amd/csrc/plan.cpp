@@ -20,7 +20,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <memory>
 
 #include "hostmath.hpp"
 
@@ -374,7 +376,6 @@ Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
 }
 
 namespace {
-BlockDecider g_decider = nullptr;
 constexpr int kMaxTerms = 256;
 }  // namespace
 
@@ -384,12 +385,46 @@ bool default_decider(const BlockChoice& c) {
   return c.single_sweeps > 1.0;
 }
 
-void set_block_decider(BlockDecider d) { g_decider = d; }
+void op_features(const Op& o, int n, int precision, float* f) {
+  for (int i = 0; i < KET_MLP_INPUTS; ++i) f[i] = 0.f;
+  const int L = line_bits(precision);
+  uint64_t qm = 0;
+  int width = 0;
+  switch (o.kind) {
+    case Kind::Block:
+    case Kind::MatrixExact:
+      for (int b = 0; b < o.k; ++b) qm |= bit(o.qubits[b]);
+      width = o.k;
+      f[o.kind == Kind::Block ? 0 : 4] = 1.f;
+      break;
+    case Kind::Pair:
+      for (int b = 0; b < o.spec.npos; ++b) qm |= bit(o.spec.pos[b]);
+      width = o.spec.npos;
+      f[o.pair_op == ketk::OP_DENSE ? 1 : o.pair_op == ketk::OP_SWAP ? 3 : 4] = 1.f;
+      break;
+    case Kind::Phase:
+      qm = o.spec.set_mask;
+      width = o.spec.npos;
+      f[2] = 1.f;
+      break;
+  }
+  const double sweep = 2.0 * (precision == 0 ? 16.0 : 8.0) * (double)(uint64_t{1} << n);
+  f[5] = width / 6.f;
+  f[6] = std::min(o.n_gates, 128) / 32.f;
+  f[7] = o.n_terms / 64.f;
+  f[8] = qm ? __builtin_ctzll(qm) / 32.f : 0.f;
+  f[9] = qm ? (63 - __builtin_clzll(qm)) / 32.f : 0.f;
+  f[10] = __builtin_popcountll(qm & (bit(L) - 1)) / 4.f;
+  f[11] = (float)o.line_fraction;
+  f[12] = (float)precision;
+  f[13] = (n - 24) / 8.f;
+  f[14] = (float)(std::log2(sweep * o.line_fraction) / 40.0);
+  f[15] = 1.f;
+}
 
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 BlockDecider decide, bool force_blocks) {
-  if (!decide) decide = g_decider ? g_decider : default_decider;
+                 const BlockDecider& decide, bool force_blocks) {
   const int L = line_bits(precision);
   const uint64_t all = n >= 64 ? ~uint64_t{0} : (bit(n) - 1);
   std::vector<std::size_t> rem(run);
@@ -440,7 +475,11 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
     order.insert(order.end(), trailing.begin(), trailing.end());
     std::sort(order.begin(), order.end());
     double sweeps = 0.0;
-    for (std::size_t gi : order) sweeps += op_for_gate(gates[gi], n, precision).line_fraction;
+    std::vector<Op> singles;
+    for (std::size_t gi : order) {
+      singles.push_back(op_for_gate(gates[gi], n, precision));
+      sweeps += singles.back().line_fraction;
+    }
     const int k = __builtin_popcountll(Q);
     int low = 0;
     for (int qb = 0; qb < L; ++qb)
@@ -448,8 +487,11 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
     BlockChoice ch{k, (int)members.size(), (int)trailing.size(), (int)terms.size(),
                    Q ? __builtin_ctzll(Q) : 0, low, sweeps, precision, n};
     const bool single_gate = order.size() == 1;
-    if (!force_blocks && (single_gate || !decide(ch))) {
-      for (std::size_t gi : order) plan->ops.push_back(op_for_gate(gates[gi], n, precision));
+    if (!force_blocks && single_gate) {
+      // Still reported, so a recording decider sees every pass of the plan
+      // and a replaying one stays aligned; the answer does not matter.
+      if (decide) (void)decide(ch, singles[0], singles);
+      plan->ops.push_back(singles[0]);
       continue;
     }
     std::vector<int> bq;
@@ -465,14 +507,93 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
       refs.push_back(r);
     }
     const std::vector<cx> mat = kethost::block_matrix(refs, k);
-    if (k == 1 && trailing.empty() && !force_blocks) {
-      plan->ops.push_back(pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data()));
-    } else {
-      plan->ops.push_back(op_for_block(*plan, n, precision, k, bq.data(), &mat[0].re, terms));
+    const std::size_t pool_mark = plan->pool.size();
+    Op block = (k == 1 && trailing.empty() && !force_blocks)
+                   ? pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data())
+                   : op_for_block(*plan, n, precision, k, bq.data(), &mat[0].re, terms);
+    block.n_gates = (int)order.size();
+    const bool use_block =
+        force_blocks || (decide ? decide(ch, block, singles) : default_decider(ch));
+    if (!use_block) {
+      plan->pool.resize(pool_mark);  // drop the unused record
+      for (const Op& o : singles) plan->ops.push_back(o);
+      continue;
     }
-    plan->ops.back().n_gates = (int)order.size();
+    plan->ops.push_back(block);
     ++plan->blocks;
   }
+}
+
+// KET_FUSE_LEARNED: for each candidate block width, pack the circuit once
+// recording every group's block pass and its gates' own passes, predict all
+// of their times with the cost model in one GPU inference, keep the width
+// whose predicted total (the cheaper option per group) is lowest, and return
+// a decider that replays the per-group choices. Without a model the greedy
+// bytes rule and the requested width stay.
+void learned_choice(const ket_gate* gates, std::size_t n_gates, int n, int precision,
+                    int* block_k, BlockDecider* decider) {
+  struct Group {
+    std::size_t block_row, single_row, n_single;
+  };
+  std::vector<float> feats;
+  std::vector<std::vector<Group>> groups;
+  std::vector<int> widths;
+  for (int k : {3, 4, 5}) {
+    if (k > n) continue;
+    widths.push_back(k);
+    groups.emplace_back();
+    std::vector<Group>& gs = groups.back();
+    Plan scratch;
+    BlockDecider rec = [&](const BlockChoice&, const Op& block, const std::vector<Op>& singles) {
+      Group g{feats.size() / KET_MLP_INPUTS, 0, singles.size()};
+      feats.resize(feats.size() + KET_MLP_INPUTS);
+      op_features(block, n, precision, &feats[feats.size() - KET_MLP_INPUTS]);
+      g.single_row = feats.size() / KET_MLP_INPUTS;
+      for (const Op& o : singles) {
+        feats.resize(feats.size() + KET_MLP_INPUTS);
+        op_features(o, n, precision, &feats[feats.size() - KET_MLP_INPUTS]);
+      }
+      gs.push_back(g);
+      return true;
+    };
+    std::vector<std::size_t> run;
+    for (std::size_t i = 0; i <= n_gates; ++i) {
+      if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
+        if (!run.empty()) pack_blocks(gates, run, n, precision, k, &scratch, rec);
+        run.clear();
+        continue;
+      }
+      run.push_back(i);
+    }
+  }
+  std::vector<float> pred;
+  if (widths.empty() || !predict_log_ms(feats, &pred)) return;
+  double best = 1e300;
+  std::size_t best_w = 0;
+  for (std::size_t w = 0; w < widths.size(); ++w) {
+    double total = 0.0;
+    for (const Group& g : groups[w]) {
+      double singles = 0.0;
+      for (std::size_t j = 0; j < g.n_single; ++j) singles += std::exp((double)pred[g.single_row + j]);
+      total += std::min(std::exp((double)pred[g.block_row]), singles);
+    }
+    if (total < best) {
+      best = total;
+      best_w = w;
+    }
+  }
+  *block_k = widths[best_w];
+  auto choice = std::make_shared<std::vector<char>>();
+  for (const Group& g : groups[best_w]) {
+    double singles = 0.0;
+    for (std::size_t j = 0; j < g.n_single; ++j) singles += std::exp((double)pred[g.single_row + j]);
+    choice->push_back(std::exp((double)pred[g.block_row]) < singles);
+  }
+  auto next = std::make_shared<std::size_t>(0);
+  *decider = [choice, next](const BlockChoice&, const Op&, const std::vector<Op>&) {
+    const std::size_t i = (*next)++;
+    return i < choice->size() ? (*choice)[i] != 0 : false;
+  };
 }
 
 int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
@@ -511,16 +632,20 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
     fuser_flush(pend, bq, n, precision, &plan);
     return KET_OK;
   }
-  if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_LEARNED) {
+  if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_LEARNED || fusion == KET_FUSE_BLOCKS_ALL) {
     if (block_k < 2 || block_k > 6) {
       *err = "ket_amd: block_k must be in [2, 6]";
       return KET_ERR_ARG;
     }
     if (block_k > n) block_k = n < 1 ? 1 : n;
+    BlockDecider decider;  // empty: bytes rule
+    if (fusion == KET_FUSE_LEARNED) learned_choice(gates, n_gates, n, precision, &block_k, &decider);
     std::vector<std::size_t> run;
     for (std::size_t i = 0; i <= n_gates; ++i) {
       if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
-        if (!run.empty()) pack_blocks(gates, run, n, precision, block_k, &plan, nullptr);
+        if (!run.empty())
+          pack_blocks(gates, run, n, precision, block_k, &plan, decider,
+                      fusion == KET_FUSE_BLOCKS_ALL);
         run.clear();
         continue;
       }

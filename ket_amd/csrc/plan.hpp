@@ -3,6 +3,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -84,15 +85,24 @@ struct BlockChoice {
   int precision;
   int n;
 };
-using BlockDecider = bool (*)(const BlockChoice&);
+// Called once per packed group with the group's block pass and its gates'
+// own passes; true emits the block.
+using BlockDecider = std::function<bool(const BlockChoice&, const Op& block,
+                                        const std::vector<Op>& singles)>;
 bool default_decider(const BlockChoice& c);
-void set_block_decider(BlockDecider d);  // nullptr restores the default
+
+// The cost model's description of a pass (KET_MLP_INPUTS floats).
+void op_features(const Op& o, int n, int precision, float* f);
+
+// Predicted log-milliseconds of feature rows with the planner's model
+// (mlp.hip); false when no model is set.
+bool predict_log_ms(const std::vector<float>& feats, std::vector<float>* out);
 
 // The k-qubit packer (KET_FUSE_BLOCKS) over the unitary gate indices `run`.
 // force_blocks: every packed group becomes a Block pass (one kernel kind
 // for the whole plan, as the batched engine needs).
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 BlockDecider decide, bool force_blocks = false);
+                 const BlockDecider& decide, bool force_blocks = false);
 
 }  // namespace ketplan

@@ -17,17 +17,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libket_amd.so")
 
 C128, C64 = 0, 1
-FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED = 0, 1, 2, 3
+FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED, FUSE_BLOCKS_ALL = 0, 1, 2, 3, 4
 
 STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: MemoryError,
           5: NotImplementedError}
 
 
+MLP_INPUTS, MLP_HIDDEN = 16, 64
+MLP_PARAMS = MLP_HIDDEN * MLP_INPUTS + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + 2 * MLP_HIDDEN + 1
+
+
 class LaunchRecord(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("k", ctypes.c_int32), ("n_gates", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("qubit_mask", ctypes.c_uint64),
+                ("n_terms", ctypes.c_int32), ("qubit_mask", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_double), ("flops", ctypes.c_double),
-                ("ms", ctypes.c_double)]
+                ("ms", ctypes.c_double), ("features", ctypes.c_float * MLP_INPUTS)]
 
 
 LAUNCH_KINDS = ["dense1q", "ch", "permute", "phase", "matrix", "block"]
@@ -98,6 +102,14 @@ def lib():
             "ket_batch_create": ([sz, vp, i32, P(vp)], i32),
             "ket_batch_destroy": ([vp], i32),
             "ket_batch_run": ([vp, vp, vp, vp, vp, vp, i32, i32, vp, P(BatchStats)], i32),
+            "ket_mlp_create": ([u64, P(vp)], i32),
+            "ket_mlp_destroy": ([vp], i32),
+            "ket_mlp_set_params": ([vp, vp, sz], i32),
+            "ket_mlp_get_params": ([vp, vp, sz], i32),
+            "ket_mlp_forward": ([vp, vp, sz, vp], i32),
+            "ket_mlp_gradients": ([vp, vp, vp, sz, vp, vp], i32),
+            "ket_mlp_train_step": ([vp, vp, vp, sz, ctypes.c_float, vp], i32),
+            "ket_mlp_use_for_planning": ([vp], i32),
             "ket_sv_event_record": ([vp, i32], i32),
             "ket_sv_event_elapsed": ([vp, i32, i32, P(ctypes.c_float)], i32),
         }
@@ -272,8 +284,9 @@ class StateVector:
         for i in range(min(nl.value, cap)):
             x = buf[i]
             out.append({"kind": LAUNCH_KINDS[x.kind], "k": x.k, "n_gates": x.n_gates,
-                        "qubit_mask": int(x.qubit_mask), "alg_bytes": x.alg_bytes,
-                        "flops": x.flops, "ms": x.ms})
+                        "n_terms": x.n_terms, "qubit_mask": int(x.qubit_mask),
+                        "alg_bytes": x.alg_bytes, "flops": x.flops, "ms": x.ms,
+                        "features": np.ctypeslib.as_array(x.features).copy()})
         return out
 
     def norm(self) -> float:
@@ -388,3 +401,59 @@ class Batch:
                                   int(block_k), 1 if use_graph else 0, _p(out),
                                   ctypes.byref(st)))
         return out, st.as_dict()
+
+
+class MLP:
+    """The planner's cost model on the GPU (ket_mlp_*)."""
+
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().ket_mlp_create(int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ket_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        check(lib().ket_mlp_get_params(self._h, _p(out), MLP_PARAMS))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().ket_mlp_set_params(self._h, _p(p), p.size))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        y = np.zeros(len(x), np.float32)
+        check(lib().ket_mlp_forward(self._h, _p(x), len(x), _p(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().ket_mlp_gradients(self._h, _p(x), _p(t), len(x), _p(g), _p(loss)))
+        return g, float(loss[0])
+
+    def train_step(self, x, t, lr: float) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().ket_mlp_train_step(self._h, _p(x), _p(t), len(x), ctypes.c_float(lr),
+                                       _p(loss)))
+        return float(loss[0])
+
+    def use_for_planning(self, on: bool = True):
+        check(lib().ket_mlp_use_for_planning(self._h if on else None))

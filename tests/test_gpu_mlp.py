@@ -1,0 +1,71 @@
+"""The cost model's GPU kernels against the float64 oracle, and the learned
+planner's plans against the state oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from ket_amd import costmodel as CM
+from ket_amd import native as N
+from ket_amd import workloads as W
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradients_match_oracle():
+    rng = np.random.default_rng(3)
+    mlp = N.MLP(seed=5)
+    p = mlp.params.astype(np.float64)
+    x = rng.normal(size=(300, N.MLP_INPUTS)).astype(np.float32)
+    t = rng.normal(size=300).astype(np.float32)
+    y = mlp.forward(x)
+    want = M.forward(p, x)
+    assert np.abs(y - want).max() <= 1e-4 * max(1.0, np.abs(want).max())
+    g, loss = mlp.gradients(x, t)
+    wl, wg = M.loss_and_grad(p, x, t)
+    assert abs(loss - wl) <= 1e-4 * max(1.0, wl)
+    assert np.linalg.norm(g - wg) <= 1e-4 * np.linalg.norm(wg)
+
+
+def test_training_fits_a_smooth_target():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, size=(4096, N.MLP_INPUTS)).astype(np.float32)
+    t = (np.sin(3 * x[:, 0]) + x[:, 1] * x[:, 2] - 0.5 * x[:, 3]).astype(np.float32)
+    mlp = N.MLP(seed=1)
+    first = mlp.train_step(x, t, 3e-3)
+    for _ in range(400):
+        last = mlp.train_step(x, t, 3e-3)
+    assert last < first / 10
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_learned_plans_keep_the_state(seed):
+    mlp = CM.load() or N.MLP(seed)
+    mlp.use_for_planning(True)
+    try:
+        for c in (W.random_mixed(12, 12, seed), W.qft_u(11, seed), W.diag_mix(10, 5, seed)):
+            want = O.run(c)
+            with N.StateVector(c.n_qubits, N.C128) as sv:
+                sv.apply_circuit(c, N.FUSE_LEARNED, 5)
+                got = sv.to_numpy()
+            assert np.abs(got - want).max() <= 1e-12
+    finally:
+        mlp.use_for_planning(False)
+
+
+@pytest.mark.skipif(not os.path.exists(CM.MODEL_PATH), reason="no trained cost model committed")
+def test_trained_model_gain_over_greedy():
+    """What the trained network gains over the greedy planner on a few
+    held-out circuits of the training distribution (reported; the learned
+    planner must not lose more than timing noise)."""
+    mlp = CM.load()
+    states = CM.States(N.C64)
+    try:
+        circs = CM.config5_circuits(6, seed=999, lo=26, hi=27, depth=40)
+        res = CM.evaluate(mlp, circs, states)
+    finally:
+        states.close()
+    print("learned vs greedy:", res)
+    assert res["speedup"] >= 0.9
