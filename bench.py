@@ -254,6 +254,56 @@ def bench_brick20(args):
     return emit(out)
 
 
+def bench_planner(args):
+    """Config 5: the learned fusion planner. Stage `train`: time windows of
+    5000 generated circuits (28-32 qubits, depth 40, 13-gate set) on the GPU
+    as per-gate passes and forced blocks, train the cost model on the GPU and
+    write it to gpurun_out/cost_model.npz. Stage `eval`: 200 held-out
+    circuits of the same distribution run whole with the greedy packer and
+    with the learned planner (complex64); wall time of each."""
+    from ket_amd import costmodel as CM
+    out = {"metric": METRIC}
+    states = CM.States(N.C64)
+    try:
+        if args.stage in ("train", "all"):
+            circs = CM.config5_circuits(args.train_circuits, seed=1234)
+            log(f"[bench] planner: timing windows of {len(circs)} training circuits")
+            t0 = time.perf_counter()
+            X, y = CM.collect(circs, states, window_len=8, seed=1, log=log)
+            t_collect = time.perf_counter() - t0
+            log(f"[bench] {len(y)} timed passes in {t_collect:.0f} s; training")
+            t0 = time.perf_counter()
+            mlp, hist = CM.train(X, y, epochs=60, log=log)
+            t_train = time.perf_counter() - t0
+            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+            CM.save(mlp, os.path.join(ROOT, "gpurun_out", "cost_model.npz"),
+                    train_passes=len(y), final_loss=hist[-1], epochs=len(hist))
+            out["train"] = {"circuits": len(circs), "passes": int(len(y)),
+                            "collect_s": t_collect, "train_s": t_train,
+                            "loss_first": hist[0], "loss_final": hist[-1]}
+        if args.stage in ("eval", "all"):
+            mlp = CM.load()
+            if mlp is None:
+                raise SystemExit("no ket_amd/cost_model.npz: run --stage train first")
+            held = CM.config5_circuits(args.eval_circuits, seed=777)
+            log(f"[bench] planner: evaluating on {len(held)} held-out circuits")
+            res = CM.evaluate(mlp, held, states, log=log)
+            out["eval"] = res
+            log(f"[bench] greedy {res['greedy_ms']:.0f} ms, learned {res['learned_ms']:.0f} ms, "
+                f"speedup {res['speedup']:.3f}")
+    finally:
+        states.close()
+    ev = out.get("eval", {})
+    out.update({
+        "value": ev.get("speedup"), "unit": "x (greedy wall time / learned wall time)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ev.get("learned_ms"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "planner: random 13-gate-set circuits, 28-32 qubits, depth 40"},
+        "device": N.device_info()["name"]})
+    return emit(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +312,9 @@ def main():
     ap.add_argument("--workload", default="qft30",
                     help="qft30 (default), brick33, hea4096, brick20, qftNN")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage", default="all", help="planner workload: train | eval | all")
+    ap.add_argument("--train-circuits", type=int, default=5000)
+    ap.add_argument("--eval-circuits", type=int, default=200)
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -273,6 +326,8 @@ def main():
 
     if args.workload == "hea4096":
         return bench_hea(args)
+    if args.workload == "planner":
+        return bench_planner(args)
     if args.workload == "brick20":
         return bench_brick20(args)
     circ, desc = workload(args.workload)
