@@ -83,6 +83,19 @@ def time_steps(sv, circ, fusion, k, steps, warmup):
     return wall, ev, dict(sv.last_stats)
 
 
+def traffic_from_profile(kind):
+    """HBM bytes per launch of the dominant kernel as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed under profiles/ (scaled
+    by the same counters on the copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "qft30_plain_c128_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {kind} "):
+            return float(line.split()[2])
+    return None
+
+
 def dominant_kernel(records):
     by = {}
     for r in records:
@@ -340,6 +353,8 @@ def main():
     out = {"metric": METRIC}
     if args.workload == "brick33":
         return bench_brick33(args, circ, desc)
+    copy_gbps = N.calibrate_copy(4 << 30, 5, 0)
+    log(f"[bench] copy kernel (4 GiB, read+write): {copy_gbps:.0f} GB/s")
     plain = N.StateVector(n, N.C128)
     log(f"[bench] plain complex128: warmup {args.warmup}, steps {args.steps}")
     wall, ev_s, stats = time_steps(plain, circ, N.FUSE_NONE, 5, args.steps, args.warmup)
@@ -355,7 +370,8 @@ def main():
     kind, d, avg_ms, avg_bytes, by = dominant_kernel(recs)
     achieved = avg_bytes / (avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic_from_profile(kind),
+                "copy_kernel_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
                 "kernel": kind, "launches": d["n"], "avg_launch_ms": avg_ms,
                 "alg_bytes_per_launch": avg_bytes,
                 "by_kind": {k: {"launches": v["n"], "ms": v["ms"],

@@ -263,9 +263,10 @@ typedef struct ket_tuning {
 } ket_tuning;
 int ket_set_tuning(const ket_tuning* t);
 int ket_get_tuning(ket_tuning* t);
-/* Achievable HBM bandwidth on this device: a plain copy of `bytes` between
- * two device buffers, `reps` times; *gbps = 2 * bytes * reps / time / 1e9
- * (read plus write). Allocates and frees the two buffers. */
+/* Achievable HBM bandwidth on this device: a plain copy kernel (the
+ * per-gate passes' launch shape) of `bytes` between two device buffers,
+ * `reps` times; *gbps = 2 * bytes * reps / time / 1e9 (read plus write).
+ * Allocates and frees the two buffers. */
 int ket_calibrate_copy(uint64_t bytes, int reps, int nontemporal, double* gbps);
 
 /* ---- timing on the state's stream ---- */

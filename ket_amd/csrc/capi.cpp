@@ -731,10 +731,7 @@ int ket_calibrate_copy(uint64_t bytes, int reps, int nontemporal, double* gbps) 
     return done(hip_fail(e, "hipStreamCreate"));
   if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
     return done(hip_fail(e, "hipEventCreate"));
-  int dev = 0, cus = 256;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const int blocks = cus * 8;
+  const int blocks = 0;  // full grid, the per-gate passes' shape
   if ((e = ketk::launch_copy(st, a, b, bytes, nontemporal, blocks)) != hipSuccess)
     return done(hip_fail(e, "copy"));
   (void)hipEventRecord(e0, st);

@@ -261,13 +261,27 @@ hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
   return hipGetLastError();
 }
 
-// Bandwidth calibration: a plain float4 copy, grid-stride.
+// Bandwidth calibration: a plain float4 copy with the per-gate passes'
+// launch shape (two 16-byte items per thread, blocks over the whole range,
+// grid-stride beyond 2^20 blocks).
 template <int NT>
 __global__ __launch_bounds__(kThreads) void k_copy(const float4* __restrict__ a,
                                                     float4* __restrict__ b, uint64_t n) {
-  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
-    st<NT>(b + i, ld<NT>(a + i));
+  const uint64_t per_block = (uint64_t)kThreads * 2;
+  for (uint64_t first = (uint64_t)blockIdx.x * per_block + threadIdx.x;
+       first - threadIdx.x < n; first += (uint64_t)gridDim.x * per_block) {
+    float4 v[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint64_t i = first + (uint64_t)j * kThreads;
+      if (i < n) v[j] = ld<NT>(a + i);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint64_t i = first + (uint64_t)j * kThreads;
+      if (i < n) st<NT>(b + i, v[j]);
+    }
+  }
 }
 
 }  // namespace
@@ -302,6 +316,11 @@ Tuning get_tuning() { return g_tuning; }
 hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes, int nt,
                        int blocks) {
   const uint64_t n = bytes / sizeof(float4);
+  if (blocks <= 0) {
+    uint64_t g = (n + 511) / 512;
+    if (g > (uint64_t{1} << 20)) g = uint64_t{1} << 20;
+    blocks = (int)(g ? g : 1);
+  }
   if (nt)
     k_copy<1><<<blocks, kThreads, 0, st>>>(static_cast<const float4*>(a),
                                            static_cast<float4*>(b), n);
