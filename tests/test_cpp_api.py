@@ -34,6 +34,13 @@ def test_api_suite_passes_on_gpu_backend():
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+@pytest.mark.gpu
+def test_amd_extensions():
+    r = run([os.path.join(ROOT, "ket_amd", "test_amd_ext")])
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def _write(c, d):
     p = os.path.join(d, "c.txt")
     with open(p, "w") as f:
