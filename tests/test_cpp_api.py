@@ -76,3 +76,23 @@ def test_driver_run_and_sample_match_reference(seed):
         assert lines[0] == "state vector"
         for p, v in zip(paulis, lines[1:]):
             assert abs(float(v) - O.expval(want, p)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_qasm_front_end():
+    qdir = os.path.join(ROOT, "tests", "qasm")
+    files = sorted(os.path.join(qdir, f) for f in os.listdir(qdir) if f.endswith(".qasm"))
+    r = run([os.path.join(ROOT, "ket_amd", "test_qasm")] + files)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = {}
+    for line in r.stdout.strip().split("\n"):
+        parts = line.split()
+        res[os.path.basename(parts[0])] = dict(p.split("=") for p in parts[1:])
+    assert res["ghz5.qasm"]["qubits"] == "5" and abs(float(res["ghz5.qasm"]["pmax"]) - 0.5) < 1e-12
+    assert res["grover3.qasm"]["argmax"] == "5"
+    assert abs(float(res["grover3.qasm"]["pmax"]) - 0.9453125) < 1e-12
+    assert abs(float(res["qft_basis.qasm"]["pmax"]) - 1 / 16) < 1e-12
+    bad = os.path.join(ROOT, "tests", "qasm", "bad.qasm.txt")
+    r = run([os.path.join(ROOT, "ket_amd", "test_qasm"), bad])
+    assert r.returncode == 1 and "error" in r.stdout
