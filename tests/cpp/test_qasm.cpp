@@ -33,7 +33,7 @@ int main(int argc, char** argv) {
         }
         diff = std::max(diff, std::abs(s[k] - t[k]));
       }
-      std::printf("%s qubits=%zu nodes=%zu argmax=%zu pmax=%.6f roundtrip=%.3g\n", argv[i],
+      std::printf("%s qubits=%zu nodes=%zu argmax=%zu pmax=%.17g roundtrip=%.3g\n", argv[i],
                   c.n_qubits(), c.dag().nodes().size(), arg, pmax, diff);
       if (diff > 1e-12) ++failures;
     } catch (const std::exception& e) {
