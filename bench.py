@@ -300,10 +300,11 @@ def bench_planner(args):
                 raise SystemExit("no ket_amd/cost_model.npz: run --stage train first")
             held = CM.config5_circuits(args.eval_circuits, seed=777)
             log(f"[bench] planner: evaluating on {len(held)} held-out circuits")
-            res = CM.evaluate(mlp, held, states, log=log)
+            res = CM.evaluate(mlp, held, states, log=log, ref_subset=args.ref_subset)
             out["eval"] = res
             log(f"[bench] greedy {res['greedy_ms']:.0f} ms, learned {res['learned_ms']:.0f} ms, "
-                f"speedup {res['speedup']:.3f}")
+                f"speedup {res['speedup']:.3f}; vs ket's Fuser on {res.get('reference_fuser_circuits')} "
+                f"circuits: {res.get('speedup_vs_reference_fuser')}")
     finally:
         states.close()
     ev = out.get("eval", {})
@@ -328,6 +329,8 @@ def main():
     ap.add_argument("--stage", default="all", help="planner workload: train | eval | all")
     ap.add_argument("--train-circuits", type=int, default=5000)
     ap.add_argument("--eval-circuits", type=int, default=200)
+    ap.add_argument("--ref-subset", type=int, default=20,
+                    help="planner eval: circuits also run with ket's own Fuser rule")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()

@@ -118,17 +118,23 @@ def time_circuit(sv: N.StateVector, c: Circuit, fusion: int, k: int) -> float:
     return sv.event_elapsed(0, 1)
 
 
-def evaluate(mlp: N.MLP, circuits, states: States, log=None) -> dict:
-    """Greedy (k = 5) vs learned planner, whole circuits, and the model's
-    relative error on the learned plans' passes."""
-    tg, tl, errs = [], [], []
+def evaluate(mlp: N.MLP, circuits, states: States, log=None, ref_subset: int = 0) -> dict:
+    """Learned planner vs the greedy packer (k = 5, bytes rule) on whole
+    circuits, the model's relative error on the learned plans' passes, and,
+    on the first `ref_subset` circuits, learned vs ket's own greedy Fuser
+    (KET_FUSE_REFERENCE, the reference's fusion rule)."""
+    tg, tl, errs, tr, tlr = [], [], [], [], []
     for i, c in enumerate(circuits):
         sv = states.get(c.n_qubits)
         mlp.use_for_planning(False)
         time_circuit(sv, c, N.FUSE_BLOCKS, 5)  # warm
         tg.append(time_circuit(sv, c, N.FUSE_BLOCKS, 5))
+        if i < ref_subset:
+            tr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
         mlp.use_for_planning(True)
         tl.append(time_circuit(sv, c, N.FUSE_LEARNED, 5))
+        if i < ref_subset:
+            tlr.append(tl[-1])
         sv.set_basis(0)
         recs = sv.profile_circuit(c, N.FUSE_LEARNED, 5)
         feats = np.array([r["features"] for r in recs], np.float32)
@@ -140,11 +146,16 @@ def evaluate(mlp: N.MLP, circuits, states: States, log=None) -> dict:
                 f"learned {sum(tl):.0f} ms")
     mlp.use_for_planning(False)
     tg, tl = np.array(tg), np.array(tl)
-    return {"circuits": len(circuits), "greedy_ms": float(tg.sum()), "learned_ms": float(tl.sum()),
-            "speedup": float(tg.sum() / tl.sum()),
-            "per_circuit_speedup_median": float(np.median(tg / tl)),
-            "learned_faster_fraction": float(np.mean(tl < tg)),
-            "model_total_time_rel_error_median": float(np.median(errs))}
+    out = {"circuits": len(circuits), "greedy_ms": float(tg.sum()), "learned_ms": float(tl.sum()),
+           "speedup": float(tg.sum() / tl.sum()),
+           "per_circuit_speedup_median": float(np.median(tg / tl)),
+           "learned_faster_fraction": float(np.mean(tl < tg)),
+           "model_total_time_rel_error_median": float(np.median(errs))}
+    if tr:
+        out.update({"reference_fuser_circuits": len(tr), "reference_fuser_ms": float(sum(tr)),
+                    "learned_ms_same_circuits": float(sum(tlr)),
+                    "speedup_vs_reference_fuser": float(sum(tr) / sum(tlr))})
+    return out
 
 
 def config5_circuits(count: int, seed: int, lo: int = 28, hi: int = 32, depth: int = 40):

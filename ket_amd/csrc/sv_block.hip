@@ -36,42 +36,66 @@ __device__ __forceinline__ uint64_t spread(uint64_t g, const int* sorted, int k)
   return g;
 }
 
-// Group constants of the trailing phase: C and A_j (j < k) for base x.
-template <int KMAX>
-__device__ __forceinline__ void group_angles(uint64_t x, const BlockRecord& h,
-                                             const uint64_t* tm, const double* th,
-                                             double& C, double (&A)[KMAX]) {
-  C = 0.0;
-  for (int t = h.term_start[0]; t < h.term_start[1]; ++t)
-    if ((x & tm[t]) == tm[t]) C += th[t];
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    A[j] = 0.0;
-    if (j < h.k)
-      for (int t = h.term_start[j + 1]; t < h.term_start[j + 2]; ++t)
-        if ((x & tm[t]) == tm[t]) A[j] += th[t];
-  }
-}
-
 __device__ __forceinline__ double reduce_2pi(double a) {
   const double two_pi = 6.283185307179586476925287;
   return a - two_pi * rint(a / two_pi);
 }
 
+// Group constants of the trailing phase: C and A_j (j < k) for base x, over
+// the terms t = first, first + step, ... of each range (the MFMA kernel
+// splits the terms of one group over the four lanes that share it).
+template <int KMAX>
+__device__ __forceinline__ void group_angles(uint64_t x, const BlockRecord& h,
+                                             const uint64_t* tm, const double* th,
+                                             double& C, double (&A)[KMAX], int first = 0,
+                                             int step = 1) {
+  C = 0.0;
+  for (int t = h.term_start[0] + first; t < h.term_start[1]; t += step)
+    if ((x & tm[t]) == tm[t]) C += th[t];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    A[j] = 0.0;
+    if (j < h.k)
+      for (int t = h.term_start[j + 1] + first; t < h.term_start[j + 2]; t += step)
+        if ((x & tm[t]) == tm[t]) A[j] += th[t];
+  }
+}
+
+// exp(i * angle) in the state's precision (FP64 range reduction first).
 template <class T>
-__device__ __forceinline__ ketd::cplx<T> rotate(ketd::cplx<T> v, double angle);
+__device__ __forceinline__ ketd::cplx<T> expi(double angle);
 template <>
-__device__ __forceinline__ double2 rotate<double>(double2 v, double angle) {
+__device__ __forceinline__ double2 expi<double>(double angle) {
   double s, c;
   sincos(reduce_2pi(angle), &s, &c);
-  return make_double2(v.x * c - v.y * s, v.x * s + v.y * c);
+  return make_double2(c, s);
 }
 template <>
-__device__ __forceinline__ float2 rotate<float>(float2 v, double angle) {
+__device__ __forceinline__ float2 expi<float>(double angle) {
   float s, c;
   sincosf((float)reduce_2pi(angle), &s, &c);
-  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+  return make_float2(c, s);
 }
+
+// The per-element phase exp(i(C + sum_j bit_j(e) A_j)) is
+// E * prod_{bits j of e} F_j with E = exp(iC), F_j = exp(iA_j): k + 1
+// sincos per group instead of one per amplitude.
+template <class T, int K>
+struct PhaseFactors {
+  ketd::cplx<T> E, F[K];
+  __device__ __forceinline__ void set(double C, const double (&A)[K]) {
+    E = expi<T>(C);
+#pragma unroll
+    for (int j = 0; j < K; ++j) F[j] = expi<T>(A[j]);
+  }
+  __device__ __forceinline__ ketd::cplx<T> at(int e) const {
+    ketd::cplx<T> p = E;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if ((e >> j) & 1) p = ketd::cmul<T>(p, F[j]);
+    return p;
+  }
+};
 
 // ---- vector path (any k <= 6, both precisions) ----
 // One thread per group; `block` and `nblocks` enumerate the workgroups that
@@ -108,8 +132,12 @@ __device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
     ketd::cplx<T> in[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) in[i] = s[base | off[i]];
-    double C = 0.0, A[K];
-    if constexpr (PH) group_angles<K>(base, h, tm, th, C, A);
+    PhaseFactors<T, K> ph;
+    if constexpr (PH) {
+      double C, A[K];
+      group_angles<K>(base, h, tm, th, C, A);
+      ph.set(C, A);
+    }
 #pragma unroll 1
     for (int r = 0; r < D; ++r) {
       T re = 0, im = 0;
@@ -122,13 +150,7 @@ __device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
         im = fma(m.y, in[c].x, im);
       }
       ketd::cplx<T> o = ketd::mk<T>(re, im);
-      if constexpr (PH) {
-        double ang = C;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-          if ((r >> j) & 1) ang += A[j];
-        o = rotate<T>(o, ang);
-      }
+      if constexpr (PH) o = ketd::cmul<T>(o, ph.at(r));
       s[base | off[r]] = o;
     }
   }
@@ -212,8 +234,26 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
       for (int rb = 0; rb < RB; ++rb)
         acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[rb], 0, 0, 0);
     }
-    double C = 0.0, A[K];
-    if constexpr (PH) group_angles<K>(base, h, tm, th, C, A);
+    PhaseFactors<float, K> ph;
+    float2 G = make_float2(1.f, 0.f);
+    if constexpr (PH) {
+      // The four lanes of a group (q = 0..3) each sum a quarter of the
+      // terms; two xor-shuffles give all four the totals.
+      double C, A[K];
+      group_angles<K>(base, h, tm, th, C, A, q, 4);
+      C += __shfl_xor(C, 16, 64);
+      C += __shfl_xor(C, 32, 64);
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        A[j] += __shfl_xor(A[j], 16, 64);
+        A[j] += __shfl_xor(A[j], 32, 64);
+      }
+      ph.set(C, A);
+      // Bits 2 and 3 of every element this lane writes are q.
+      G = ph.E;
+      if (q & 1) G = ketd::cmul<float>(G, ph.F[2]);
+      if (q & 2) G = ketd::cmul<float>(G, ph.F[3]);
+    }
 #pragma unroll
     for (int rb = 0; rb < RB / 2; ++rb)
 #pragma unroll
@@ -221,11 +261,13 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
         const int e = rb * 16 + 4 * q + v;
         float2 o = make_float2(acc[rb][v], acc[rb + RB / 2][v]);
         if constexpr (PH) {
-          double ang = C;
+          float2 p = G;
+          if (v & 1) p = ketd::cmul<float>(p, ph.F[0]);
+          if (v & 2) p = ketd::cmul<float>(p, ph.F[1]);
 #pragma unroll
-          for (int j = 0; j < K; ++j)
-            if ((e >> j) & 1) ang += A[j];
-          o = rotate<float>(o, ang);
+          for (int j = 4; j < K; ++j)
+            if ((rb >> (j - 4)) & 1) p = ketd::cmul<float>(p, ph.F[j]);
+          o = ketd::cmul<float>(o, p);
         }
         s[base | off[e]] = o;
       }
