@@ -259,7 +259,10 @@ typedef struct ket_tuning {
   int32_t nontemporal;      /* 1: streaming (nontemporal) loads and stores */
   int32_t blocks_per_cu;    /* 0: one block per 256*items items; else cap the
                                grid at 256*blocks_per_cu blocks (grid-stride) */
-  int32_t pad;
+  int32_t block_tiles;      /* MFMA blocks: 16-group tiles each wave loads
+                               before computing (1, 2; 0 = default 1) */
+  int32_t block_bpc;        /* fused blocks: workgroups per CU of the capped
+                               grid (MFMA: bpc, vector: 2*bpc; 0 = 8) */
 } ket_tuning;
 int ket_set_tuning(const ket_tuning* t);
 int ket_get_tuning(ket_tuning* t);

@@ -17,6 +17,8 @@ import json
 import os
 import sys
 
+import numpy as np
+
 from . import native as N
 from .gates import Circuit
 
@@ -42,6 +44,52 @@ def probe_circuit(n: int) -> Circuit:
     c.swap(3, hi)
     c.ccx(lo, mid, hi - 1)
     return c
+
+
+BLOCK_SPACE = {"block_tiles": [1, 2], "block_bpc": [4, 8, 16, 32]}
+
+
+def block_probe(n: int, k: int) -> Circuit:
+    """Fused blocks on low, middle and high qubits, each with trailing
+    controlled phases to qubits outside it."""
+    c = Circuit(n)
+    for lo in (0, n // 2 - k // 2, n - k):
+        qs = list(range(lo, lo + k))
+        for q in qs:
+            c.u(q, 0.3, 0.2, 0.1)
+        for a, b in zip(qs, qs[1:]):
+            c.cx(a, b)
+        for q in qs:
+            c.cp(q, (q + k + 3) % n if (q + k + 3) % n not in qs else (q + 2 * k) % n, 0.4)
+        c.barrier()
+    return c
+
+
+def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -> dict:
+    k = 5 if precision == N.C64 else 4
+    circ = block_probe(n, k)
+    base = N.get_tuning()
+    results = []
+    with N.StateVector(n, precision) as sv:
+        for bt, bpc in itertools.product(*BLOCK_SPACE.values()):
+            N.set_tuning(base["items_per_thread"], base["nontemporal"], base["blocks_per_cu"],
+                         bt, bpc)
+            sv.profile_circuit(circ, N.FUSE_BLOCKS_ALL, k)
+            ms = []
+            for _ in range(reps):
+                recs = [r for r in sv.profile_circuit(circ, N.FUSE_BLOCKS_ALL, k)
+                        if r["kind"] == "block"]
+                ms.append(sum(r["ms"] for r in recs))
+            bytes_ = sum(r["alg_bytes"] for r in recs)
+            m = {"block_tiles": bt, "block_bpc": bpc, "ms": float(np.median(ms)),
+                 "GBps": bytes_ / (float(np.median(ms)) / 1e3) / 1e9}
+            results.append(m)
+            log(f"[tune] blocks k={k} tiles={bt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s")
+    best = min(results, key=lambda r: r["ms"])
+    N.set_tuning(base["items_per_thread"], base["nontemporal"], base["blocks_per_cu"],
+                 best["block_tiles"], best["block_bpc"])
+    return {"k": k, "best": {kk: best[kk] for kk in BLOCK_SPACE}, "best_GBps": best["GBps"],
+            "all": results}
 
 
 def measure(sv, circ, reps: int) -> dict:
@@ -95,14 +143,18 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--blocks-only", action="store_true")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     for nt in (0, 1):
         log(f"[tune] copy kernel nt={nt}: {N.calibrate_copy(4 << 30, 10, nt):.0f} GB/s")
     table = {}
     for prec, key in ((N.C128, "c128"), (N.C64, "c64")):
-        table[key] = tune(args.qubits, prec, args.reps, log)
-        log(f"[tune] {key} best {table[key]['best']} {table[key]['best_GBps']:.0f} GB/s")
+        if not args.blocks_only:
+            table[key] = tune(args.qubits, prec, args.reps, log)
+            log(f"[tune] {key} best {table[key]['best']} {table[key]['best_GBps']:.0f} GB/s")
+        table[key + "_blocks"] = tune_blocks(args.qubits, prec, args.reps, log)
+        log(f"[tune] {key} blocks best {table[key + '_blocks']['best']}")
     out = json.dumps(table, indent=1)
     if args.out:
         with open(args.out, "w") as f:

@@ -696,7 +696,10 @@ int ket_set_tuning(const ket_tuning* t) {
     return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 2, 4 or 8");
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
     return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
-  ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu});
+  if (t->block_tiles < 0 || t->block_tiles > 2 || t->block_bpc < 0 || t->block_bpc > 64)
+    return fail(KET_ERR_ARG, "ket_amd: block_tiles must be 0..2, block_bpc 0..64");
+  ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu,
+                    t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 8});
   return KET_OK;
 }
 
@@ -706,7 +709,8 @@ int ket_get_tuning(ket_tuning* t) {
   t->items_per_thread = k.items_per_thread;
   t->nontemporal = k.nontemporal;
   t->blocks_per_cu = k.blocks_per_cu;
-  t->pad = 0;
+  t->block_tiles = k.block_tiles;
+  t->block_bpc = k.block_bpc;
   return KET_OK;
 }
 

@@ -29,6 +29,8 @@ struct Tuning {
   int items_per_thread;
   int nontemporal;
   int blocks_per_cu;
+  int block_tiles;   // MFMA block kernel: 16-group tiles per wave iteration (1, 2)
+  int block_bpc;     // block kernels: workgroups per CU (grid-stride beyond)
 };
 void set_tuning(const Tuning& t);
 Tuning get_tuning();

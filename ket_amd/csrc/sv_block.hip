@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void k_block_vec_batched(
 }
 
 // ---- MFMA path (complex64, k = 4 or 5) ----
-template <int K, bool PH>
+template <int K, bool PH, int TPI>
 __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
                                                           uint64_t n_groups,
                                                           const unsigned char* __restrict__ rec) {
@@ -219,58 +219,69 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
   const uint64_t n_tiles = n_groups / 16;
   const uint64_t wave = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   const uint64_t n_waves = (uint64_t)gridDim.x * (kThreads / 64);
-  for (uint64_t tile = wave; tile < n_tiles; tile += n_waves) {
-    const uint64_t base = spread(tile * 16 + col, sorted, K);
-    float2 in[NE];
+  // TPI tiles per iteration: all their loads are issued before any MFMA, so
+  // a wave keeps TPI * 4 KB (k = 5) in flight.
+  for (uint64_t tile0 = wave * TPI; tile0 < n_tiles; tile0 += n_waves * TPI) {
+    uint64_t base[TPI];
+    float2 in[TPI][NE];
 #pragma unroll
-    for (int i = 0; i < NE; ++i) in[i] = s[base | off[q + 4 * i]];
-    floatx4 acc[RB];
+    for (int u = 0; u < TPI; ++u) {
+      const uint64_t tile = tile0 + u < n_tiles ? tile0 + u : tile0;
+      base[u] = spread(tile * 16 + col, sorted, K);
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const float b = ks < NE ? in[ks].x : in[ks - NE].y;
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[rb], 0, 0, 0);
+      for (int i = 0; i < NE; ++i) in[u][i] = s[base[u] | off[q + 4 * i]];
     }
-    PhaseFactors<float, K> ph;
-    float2 G = make_float2(1.f, 0.f);
-    if constexpr (PH) {
-      // The four lanes of a group (q = 0..3) each sum a quarter of the
-      // terms; two xor-shuffles give all four the totals.
-      double C, A[K];
-      group_angles<K>(base, h, tm, th, C, A, q, 4);
-      C += __shfl_xor(C, 16, 64);
-      C += __shfl_xor(C, 32, 64);
 #pragma unroll
-      for (int j = 0; j < K; ++j) {
-        A[j] += __shfl_xor(A[j], 16, 64);
-        A[j] += __shfl_xor(A[j], 32, 64);
+    for (int u = 0; u < TPI; ++u) {
+      if (u > 0 && tile0 + u >= n_tiles) break;
+      floatx4 acc[RB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const float b = ks < NE ? in[u][ks].x : in[u][ks - NE].y;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[rb], 0, 0, 0);
       }
-      ph.set(C, A);
-      // Bits 2 and 3 of every element this lane writes are q.
-      G = ph.E;
-      if (q & 1) G = ketd::cmul<float>(G, ph.F[2]);
-      if (q & 2) G = ketd::cmul<float>(G, ph.F[3]);
-    }
+      PhaseFactors<float, K> ph;
+      float2 G = make_float2(1.f, 0.f);
+      if constexpr (PH) {
+        // The four lanes of a group (q = 0..3) each sum a quarter of the
+        // terms; two xor-shuffles give all four the totals.
+        double C, A[K];
+        group_angles<K>(base[u], h, tm, th, C, A, q, 4);
+        C += __shfl_xor(C, 16, 64);
+        C += __shfl_xor(C, 32, 64);
 #pragma unroll
-    for (int rb = 0; rb < RB / 2; ++rb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int e = rb * 16 + 4 * q + v;
-        float2 o = make_float2(acc[rb][v], acc[rb + RB / 2][v]);
-        if constexpr (PH) {
-          float2 p = G;
-          if (v & 1) p = ketd::cmul<float>(p, ph.F[0]);
-          if (v & 2) p = ketd::cmul<float>(p, ph.F[1]);
-#pragma unroll
-          for (int j = 4; j < K; ++j)
-            if ((rb >> (j - 4)) & 1) p = ketd::cmul<float>(p, ph.F[j]);
-          o = ketd::cmul<float>(o, p);
+        for (int j = 0; j < K; ++j) {
+          A[j] += __shfl_xor(A[j], 16, 64);
+          A[j] += __shfl_xor(A[j], 32, 64);
         }
-        s[base | off[e]] = o;
+        ph.set(C, A);
+        // Bits 2 and 3 of every element this lane writes are q.
+        G = ph.E;
+        if (q & 1) G = ketd::cmul<float>(G, ph.F[2]);
+        if (q & 2) G = ketd::cmul<float>(G, ph.F[3]);
       }
+#pragma unroll
+      for (int rb = 0; rb < RB / 2; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int e = rb * 16 + 4 * q + v;
+          float2 o = make_float2(acc[rb][v], acc[rb + RB / 2][v]);
+          if constexpr (PH) {
+            float2 p = G;
+            if (v & 1) p = ketd::cmul<float>(p, ph.F[0]);
+            if (v & 2) p = ketd::cmul<float>(p, ph.F[1]);
+#pragma unroll
+            for (int j = 4; j < K; ++j)
+              if ((rb >> (j - 4)) & 1) p = ketd::cmul<float>(p, ph.F[j]);
+            o = ketd::cmul<float>(o, p);
+          }
+          s[base[u] | off[e]] = o;
+        }
+    }
   }
 }
 
@@ -283,7 +294,8 @@ inline unsigned capped_grid(uint64_t items, uint64_t cap) {
 
 template <class T, bool PH>
 hipError_t block_vec(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
-  const unsigned g = capped_grid(groups, 256 * 16);
+  const Tuning t = get_tuning();
+  const unsigned g = capped_grid(groups, 256 * (uint64_t)(t.block_bpc > 0 ? 2 * t.block_bpc : 16));
   auto* p = static_cast<ketd::cplx<T>*>(s);
   switch (k) {
     case 1: k_block_vec<T, 1, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
@@ -299,17 +311,24 @@ hipError_t block_vec(hipStream_t st, void* s, uint64_t groups, int k, const unsi
 
 template <bool PH>
 hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
-  // Waves stream tiles of 16 groups; a grid of 8 blocks (32 waves) per CU
-  // keeps the matrix registers loaded once per wave.
+  // Waves stream tiles of 16 groups; a capped grid (block_bpc workgroups
+  // per CU) keeps the matrix registers loaded once per wave.
+  const Tuning t = get_tuning();
+  const int tpi = t.block_tiles == 2 ? 2 : 1;
   const uint64_t tiles = groups / 16;
-  uint64_t blocks = (tiles + 3) / 4;
-  if (blocks > 256 * 8) blocks = 256 * 8;
+  uint64_t blocks = (tiles + 4 * tpi - 1) / (4 * tpi);
+  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 8);
+  if (blocks > cap) blocks = cap;
   if (blocks == 0) blocks = 1;
   auto* p = static_cast<float2*>(s);
-  if (k == 4)
-    k_block_mfma<4, PH><<<(unsigned)blocks, kThreads, 0, st>>>(p, groups, rec);
-  else
-    k_block_mfma<5, PH><<<(unsigned)blocks, kThreads, 0, st>>>(p, groups, rec);
+  const unsigned b = (unsigned)blocks;
+  if (k == 4) {
+    if (tpi == 2) k_block_mfma<4, PH, 2><<<b, kThreads, 0, st>>>(p, groups, rec);
+    else k_block_mfma<4, PH, 1><<<b, kThreads, 0, st>>>(p, groups, rec);
+  } else {
+    if (tpi == 2) k_block_mfma<5, PH, 2><<<b, kThreads, 0, st>>>(p, groups, rec);
+    else k_block_mfma<5, PH, 1><<<b, kThreads, 0, st>>>(p, groups, rec);
+  }
   return hipGetLastError();
 }
 

@@ -140,11 +140,12 @@ def device_info() -> dict:
 
 class Tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("block_tiles", ctypes.c_int32),
+                ("block_bpc", ctypes.c_int32)]
 
 
-def set_tuning(items_per_thread=4, nontemporal=0, blocks_per_cu=0):
-    t = Tuning(items_per_thread, nontemporal, blocks_per_cu, 0)
+def set_tuning(items_per_thread=2, nontemporal=0, blocks_per_cu=0, block_tiles=0, block_bpc=0):
+    t = Tuning(items_per_thread, nontemporal, blocks_per_cu, block_tiles, block_bpc)
     check(lib().ket_set_tuning(ctypes.byref(t)))
 
 
@@ -152,7 +153,8 @@ def get_tuning() -> dict:
     t = Tuning()
     check(lib().ket_get_tuning(ctypes.byref(t)))
     return {"items_per_thread": t.items_per_thread, "nontemporal": t.nontemporal,
-            "blocks_per_cu": t.blocks_per_cu}
+            "blocks_per_cu": t.blocks_per_cu, "block_tiles": t.block_tiles,
+            "block_bpc": t.block_bpc}
 
 
 def calibrate_copy(nbytes: int = 4 << 30, reps: int = 10, nontemporal: int = 0) -> float:
