@@ -82,9 +82,13 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
                 ms.append(sum(r["ms"] for r in recs))
             bytes_ = sum(r["alg_bytes"] for r in recs)
             m = {"block_tiles": bt, "block_bpc": bpc, "ms": float(np.median(ms)),
-                 "GBps": bytes_ / (float(np.median(ms)) / 1e3) / 1e9}
+                 "GBps": bytes_ / (float(np.median(ms)) / 1e3) / 1e9,
+                 "per_block_GBps": {f"q{(r['qubit_mask'] & -r['qubit_mask']).bit_length() - 1}":
+                                    round(r["alg_bytes"] / (r["ms"] / 1e3) / 1e9)
+                                    for r in recs}}
             results.append(m)
-            log(f"[tune] blocks k={k} tiles={bt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s")
+            log(f"[tune] blocks k={k} tiles={bt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
+                f"by lowest qubit {m['per_block_GBps']}")
     best = min(results, key=lambda r: r["ms"])
     N.set_tuning(base["items_per_thread"], base["nontemporal"], base["blocks_per_cu"],
                  best["block_tiles"], best["block_bpc"])
