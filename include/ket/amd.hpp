@@ -4,7 +4,8 @@
 // ket's own fusion rule. The environment can set them too:
 //   KET_AMD_PRECISION = c128 | c64
 //   KET_AMD_FUSION    = reference | none | blocks | learned
-//   KET_AMD_BLOCK_K   = 2..6 (width of fused blocks)
+//   KET_AMD_BLOCK_K   = 2..6 (width of fused blocks; default 4 for complex128,
+//                       5 for complex64)
 #pragma once
 
 #include <cstddef>
@@ -26,7 +27,7 @@ enum class Fusion { None, Reference, Blocks, Learned };
 struct Options {
   Precision precision = Precision::Complex128;
   Fusion fusion = Fusion::Reference;
-  int block_k = 5;
+  int block_k = 0;  // 0: 4 for complex128, 5 for complex64
 };
 
 void set_options(const Options& options);

@@ -131,8 +131,8 @@ int ket_sv_apply_matrix(ket_sv* sv, int k, const int* qubits,
                         const double* mat);
 
 /* ---- circuits: apply_circuit + Fuser (src/simulator.cpp:400-517) ---- */
-/* block_k: largest fused block for KET_FUSE_BLOCKS/LEARNED (2..6). stats may
- * be NULL. */
+/* block_k: largest fused block for KET_FUSE_BLOCKS/LEARNED (2..6; 0: 4 for
+ * complex128, 5 for complex64). stats may be NULL. */
 int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
                          int fusion, int block_k, ket_run_stats* stats);
 
@@ -262,7 +262,12 @@ typedef struct ket_tuning {
   int32_t block_tiles;      /* MFMA blocks: 16-group tiles each wave loads
                                before computing (1, 2; 0 = default 1) */
   int32_t block_bpc;        /* fused blocks: workgroups per CU of the capped
-                               grid (MFMA: bpc, vector: 2*bpc; 0 = 8) */
+                               grid (MFMA: bpc, vector: 2*bpc; 0 = 32) */
+  int32_t block_stage;      /* LDS tiles for fused blocks (16-byte accesses in
+                               index order): complex64 blocks of width 4, 5
+                               whenever this is > 0, other blocks of width
+                               3..5 whose lowest qubit is below it (0: never;
+                               64: always; default 3) */
 } ket_tuning;
 int ket_set_tuning(const ket_tuning* t);
 int ket_get_tuning(ket_tuning* t);

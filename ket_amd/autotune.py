@@ -72,8 +72,7 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
     results = []
     with N.StateVector(n, precision) as sv:
         for bt, bpc in itertools.product(*BLOCK_SPACE.values()):
-            N.set_tuning(base["items_per_thread"], base["nontemporal"], base["blocks_per_cu"],
-                         bt, bpc)
+            N.set_tuning(block_tiles=bt, block_bpc=bpc)
             sv.profile_circuit(circ, N.FUSE_BLOCKS_ALL, k)
             ms = []
             for _ in range(reps):
@@ -90,8 +89,7 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
             log(f"[tune] blocks k={k} tiles={bt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 f"by lowest qubit {m['per_block_GBps']}")
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(base["items_per_thread"], base["nontemporal"], base["blocks_per_cu"],
-                 best["block_tiles"], best["block_bpc"])
+    N.set_tuning(block_tiles=best["block_tiles"], block_bpc=best["block_bpc"])
     return {"k": k, "best": {kk: best[kk] for kk in BLOCK_SPACE}, "best_GBps": best["GBps"],
             "all": results}
 
@@ -116,14 +114,15 @@ def tune(n: int = 30, precision: int = N.C128, reps: int = 3, log=print) -> dict
     results = []
     with N.StateVector(n, precision) as sv:
         for ipt, nt, bpc in itertools.product(*SPACE.values()):
-            N.set_tuning(ipt, nt, bpc)
+            N.set_tuning(items_per_thread=ipt, nontemporal=nt, blocks_per_cu=bpc)
             m = measure(sv, circ, reps)
             m.update({"items_per_thread": ipt, "nontemporal": nt, "blocks_per_cu": bpc})
             results.append(m)
             log(f"[tune] ipt={ipt} nt={nt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 + " ".join(f"{k}={v:.0f}" for k, v in m["by_kind_GBps"].items()))
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(best["items_per_thread"], best["nontemporal"], best["blocks_per_cu"])
+    N.set_tuning(items_per_thread=best["items_per_thread"], nontemporal=best["nontemporal"],
+                 blocks_per_cu=best["blocks_per_cu"])
     return {"device": N.device_info()["name"], "precision": precision, "qubits": n,
             "best": {k: best[k] for k in SPACE}, "best_GBps": best["GBps"], "all": results}
 
@@ -137,7 +136,8 @@ def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
     entry = table.get("c128" if precision == N.C128 else "c64")
     if entry:
         b = entry["best"]
-        N.set_tuning(b["items_per_thread"], b["nontemporal"], b["blocks_per_cu"])
+        N.set_tuning(items_per_thread=b["items_per_thread"], nontemporal=b["nontemporal"],
+                     blocks_per_cu=b["blocks_per_cu"])
         return b
     return None
 

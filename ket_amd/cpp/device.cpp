@@ -172,8 +172,8 @@ double draw_uniform(std::mt19937& rng) {
 namespace ket::amd {
 
 void set_options(const Options& options) {
-  if (options.block_k < 2 || options.block_k > 6)
-    throw std::invalid_argument("ket::amd: block_k must be in [2, 6]");
+  if (options.block_k != 0 && (options.block_k < 2 || options.block_k > 6))
+    throw std::invalid_argument("ket::amd: block_k must be 0 or in [2, 6]");
   std::lock_guard<std::mutex> lk(detail::g_opt_mu);
   detail::g_opt_init = true;
   detail::g_opt = options;

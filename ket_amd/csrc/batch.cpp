@@ -69,6 +69,7 @@ struct Launch {
   // Lds: offsets of the arrays in the pool
   std::size_t nq_off = 0, ops_off = 0, opoff_off = 0, z_off = 0, c_off = 0, toff_off = 0;
   int expval_terms = 0;
+  int lowest = -1;             // BlockSingle: the block's lowest qubit
 };
 
 std::size_t align(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
@@ -208,7 +209,8 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
                   int block_k, int use_graph, double* out, ket_batch_stats* stats) {
   if (!b) return fail(KET_ERR_ARG, "ket_amd: null batch");
   if (!gate_off || !term_off || !out) return fail(KET_ERR_ARG, "ket_amd: null batch arrays");
-  if (block_k < 1 || block_k > 6) return fail(KET_ERR_ARG, "ket_amd: block_k must be in [1, 6]");
+  if (block_k == 0) block_k = ketplan::default_block_k(b->precision);
+  if (block_k < 1 || block_k > 6) return fail(KET_ERR_ARG, "ket_amd: block_k must be 0 or in [1, 6]");
   const std::size_t ni = b->n.size();
   const auto t0 = std::chrono::steady_clock::now();
   const int P = b->precision;
@@ -342,6 +344,7 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
           L.kind = Launch::BlockSingle;
           L.n = n;
           L.k = o.k;
+          L.lowest = ketplan::lowest_qubit(o);
           L.n_terms = o.n_terms;
           L.instances = 1;
           L.arena_off = base + (x << n);
@@ -439,7 +442,8 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
                                          L.rec_stride, L.instances);
           break;
         case Launch::BlockSingle:
-          e = ketk::launch_block(b->stream, P, arena, L.n, L.k, L.n_terms, pd + L.pool_off);
+          e = ketk::launch_block(b->stream, P, arena, L.n, L.k, L.n_terms, pd + L.pool_off,
+                                 L.lowest);
           break;
         case Launch::Expval:
           e = ketk::launch_expval_diag_batched(b->stream, P, arena, L.n, L.instances,

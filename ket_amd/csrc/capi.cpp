@@ -130,7 +130,7 @@ int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
       break;
     case ketplan::Kind::Block:
       e = ketk::launch_block(sv->stream, sv->precision, sv->data, sv->n, o.k,
-                             o.n_terms, pool_dev + o.pool_offset);
+                             o.n_terms, pool_dev + o.pool_offset, ketplan::lowest_qubit(o));
       break;
   }
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
@@ -698,8 +698,11 @@ int ket_set_tuning(const ket_tuning* t) {
     return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
   if (t->block_tiles < 0 || t->block_tiles > 2 || t->block_bpc < 0 || t->block_bpc > 64)
     return fail(KET_ERR_ARG, "ket_amd: block_tiles must be 0..2, block_bpc 0..64");
+  if (t->block_stage < 0 || t->block_stage > 64)
+    return fail(KET_ERR_ARG, "ket_amd: block_stage must be in [0, 64]");
   ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu,
-                    t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 8});
+                    t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
+                    t->block_stage});
   return KET_OK;
 }
 
@@ -711,6 +714,7 @@ int ket_get_tuning(ket_tuning* t) {
   t->blocks_per_cu = k.blocks_per_cu;
   t->block_tiles = k.block_tiles;
   t->block_bpc = k.block_bpc;
+  t->block_stage = k.block_stage;
   return KET_OK;
 }
 

@@ -31,6 +31,7 @@ struct Tuning {
   int blocks_per_cu;
   int block_tiles;   // MFMA block kernel: 16-group tiles per wave iteration (1, 2)
   int block_bpc;     // block kernels: workgroups per CU (grid-stride beyond)
+  int block_stage;   // blocks whose lowest qubit is below this go through LDS tiles
 };
 void set_tuning(const Tuning& t);
 Tuning get_tuning();
@@ -71,9 +72,10 @@ struct BlockRecord {
 
 // Fused block with fused multiply-adds; complex64 k = 4, 5 runs on MFMA.
 // rec_dev: device copy of a BlockRecord with its terms and its matrix in the
-// state's precision; n_terms as recorded in it.
+// state's precision; n_terms as recorded in it; lowest: the block's lowest
+// qubit (-1: unknown), which picks direct or LDS-staged access.
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
-                        int n_terms, const void* rec_dev);
+                        int n_terms, const void* rec_dev, int lowest);
 
 // The same block pass on `instances` states of n qubits laid out back to
 // back (instance i at s + i * 2^n amplitudes), instance i's record at

@@ -633,8 +633,9 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
     return KET_OK;
   }
   if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_LEARNED || fusion == KET_FUSE_BLOCKS_ALL) {
+    if (block_k == 0) block_k = default_block_k(precision);
     if (block_k < 2 || block_k > 6) {
-      *err = "ket_amd: block_k must be in [2, 6]";
+      *err = "ket_amd: block_k must be 0 or in [2, 6]";
       return KET_ERR_ARG;
     }
     if (block_k > n) block_k = n < 1 ? 1 : n;

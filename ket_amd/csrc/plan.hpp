@@ -49,6 +49,17 @@ struct Plan {
 // Width of a 128-byte line in index bits: 3 for complex128, 4 for complex64.
 inline int line_bits(int precision) { return precision == 0 ? 3 : 4; }
 
+// Block width when the caller passes 0: complex128 blocks of 5 qubits are
+// bound by FP64 FMA throughput rather than HBM, complex64 ones run on MFMA.
+inline int default_block_k(int precision) { return precision == 0 ? 4 : 5; }
+
+// Lowest qubit of a MatrixExact or Block op.
+inline int lowest_qubit(const Op& o) {
+  int q = o.qubits[0];
+  for (int i = 1; i < o.k; ++i) q = o.qubits[i] < q ? o.qubits[i] : q;
+  return q;
+}
+
 // Validates a gate record against an n-qubit state. Returns KET_OK or an
 // error status with `err` filled.
 int validate(const ket_gate& g, int n, std::string* err);

@@ -38,7 +38,8 @@ __device__ __forceinline__ uint64_t spread(uint64_t g, const int* sorted, int k)
 
 __device__ __forceinline__ double reduce_2pi(double a) {
   const double two_pi = 6.283185307179586476925287;
-  return a - two_pi * rint(a / two_pi);
+  const double inv_two_pi = 0.15915494309189533576888;
+  return a - two_pi * rint(a * inv_two_pi);
 }
 
 // Group constants of the trailing phase: C and A_j (j < k) for base x, over
@@ -59,6 +60,17 @@ __device__ __forceinline__ void group_angles(uint64_t x, const BlockRecord& h,
       for (int t = h.term_start[j + 1] + first; t < h.term_start[j + 2]; t += step)
         if ((x & tm[t]) == tm[t]) A[j] += th[t];
   }
+}
+
+// Angle of term range r (0: C, j + 1: A_j) for base x over the terms
+// t = first, first + step, ... of the range.
+__device__ __forceinline__ double range_angle(uint64_t x, const BlockRecord& h,
+                                              const uint64_t* tm, const double* th, int r,
+                                              int first, int step) {
+  double a = 0.0;
+  for (int t = h.term_start[r] + first; t < h.term_start[r + 1]; t += step)
+    if ((x & tm[t]) == tm[t]) a += th[t];
+  return a;
 }
 
 // exp(i * angle) in the state's precision (FP64 range reduction first).
@@ -157,7 +169,8 @@ __device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
 }
 
 template <class T, int K, bool PH>
-__global__ __launch_bounds__(kThreads) void k_block_vec(ketd::cplx<T>* __restrict__ s,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 ? 2 : 1)))
+void k_block_vec(ketd::cplx<T>* __restrict__ s,
                                                          uint64_t n_groups,
                                                          const unsigned char* __restrict__ rec) {
   block_vec_body<T, K, PH>(s, n_groups, rec, blockIdx.x, gridDim.x);
@@ -285,6 +298,392 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
   }
 }
 
+// ---- LDS-staged tiles ----
+// A tile is the 2^TB amplitudes whose indices vary in P -- the block's qubits
+// Q plus the TB-K lowest qubits outside Q -- with every other index bit
+// fixed; bit i of the tile index is the i-th lowest position of P. A
+// workgroup moves a tile between HBM and LDS with 16-byte accesses in tile
+// index order (runs of at least 2^(TB-K) consecutive amplitudes, wherever Q
+// lies), and its threads apply the block to the tile's groups from LDS. The
+// direct kernels above let each lane load its own group, which touches one
+// cache line per lane and instruction when Q holds the lowest qubits.
+//
+// LDS position = tile index ^ h(tile index): h copies the group-index bits
+// of the SWB lowest group positions that lie at or above SWB onto the Q
+// positions below SWB, so 2^SWB consecutive groups read 2^SWB distinct slots
+// of one 128-byte LDS row, while 16-byte accesses in index order stay
+// permutations within a row. h is linear over XOR, as is the deposit of an
+// index into P, so both are built from per-bit bases.
+struct TileMap {
+  uint64_t pmask;   // P (state index positions)
+  uint32_t qloc;    // tile-index positions of Q
+  uint32_t nloc;    // tile-index positions of the group index
+  uint32_t sw_src;  // swizzle source positions, 5 bits each
+  uint32_t sw_dst;  // swizzle destination positions, 5 bits each
+  int sw_n;
+};
+
+// Bits of v, lowest first, to the set positions of mask (a parallel deposit).
+__device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask) {
+  uint64_t r = 0;
+  for (; mask; mask &= mask - 1, v >>= 1)
+    if (v & 1) r |= mask & (~mask + 1);
+  return r;
+}
+
+// Bits of v, lowest first, to the positions outside mask.
+__device__ __forceinline__ uint64_t deposit_outside(uint64_t v, uint64_t mask) {
+  for (; mask; mask &= mask - 1) {
+    const int p = __builtin_ctzll(mask);
+    v = ((v >> p) << (p + 1)) | (v & ((uint64_t{1} << p) - 1));
+  }
+  return v;
+}
+
+template <int K, int TB, int SWB>
+__device__ __forceinline__ TileMap tile_map(const BlockRecord& h) {
+  static_assert(TB - K >= SWB, "tile too small for the swizzle");
+  TileMap m{};
+  uint64_t qmask = 0;
+#pragma unroll
+  for (int b = 0; b < K; ++b) qmask |= uint64_t{1} << h.sorted[b];
+  m.pmask = qmask;
+  int need = TB - K;
+  for (int p = 0; need > 0; ++p)
+    if (!((qmask >> p) & 1)) {
+      m.pmask |= uint64_t{1} << p;
+      --need;
+    }
+  int i = 0;
+  for (uint64_t r = m.pmask; r; r &= r - 1, ++i)
+    if (qmask & r & (~r + 1)) m.qloc |= 1u << i;
+  m.nloc = ((1u << TB) - 1) & ~m.qloc;
+  int ns = 0, nd = 0, seen = 0;
+  for (int p = 0; p < TB && seen < SWB; ++p)
+    if ((m.nloc >> p) & 1) {
+      if (p >= SWB) m.sw_src |= (uint32_t)p << (5 * ns++);
+      ++seen;
+    }
+  for (int p = 0; p < SWB; ++p)
+    if ((m.qloc >> p) & 1) m.sw_dst |= (uint32_t)p << (5 * nd++);
+  m.sw_n = ns;  // == nd: the group positions below SWB fill the rest
+  return m;
+}
+
+__device__ __forceinline__ uint32_t swz(uint32_t t, const TileMap& m) {
+  uint32_t x = 0;
+  for (int j = 0; j < m.sw_n; ++j)
+    x |= ((t >> ((m.sw_src >> (5 * j)) & 31)) & 1u) << ((m.sw_dst >> (5 * j)) & 31);
+  return t ^ x;
+}
+
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Moves a tile between HBM and LDS: NT threads, S 16-byte accesses each;
+// access j of thread tid covers tile indices (j * NT + tid) * EPA + [0, EPA).
+template <class T, int TB, int NT>
+struct TileIO {
+  static constexpr int EPA = 16 / (int)sizeof(ketd::cplx<T>);
+  static constexpr int S = (1 << TB) / (EPA * NT);
+  static constexpr int SB = ilog2(S);
+  static constexpr int LS = ilog2(NT * EPA);
+  using V = typename Vec16<T>::type;
+  uint64_t g0;  // deposit of this thread's first index into P
+  uint32_t l0;  // its LDS position
+  uint64_t gb[SB > 0 ? SB : 1];
+  uint32_t lb[SB > 0 ? SB : 1];
+
+  __device__ __forceinline__ void init(const TileMap& m, uint32_t tid) {
+    const uint32_t t = tid * EPA;
+    g0 = deposit(t, m.pmask);
+    l0 = swz(t, m);
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      gb[b] = deposit(uint64_t{1} << (LS + b), m.pmask);
+      lb[b] = swz(1u << (LS + b), m);
+    }
+  }
+  __device__ __forceinline__ uint64_t gaddr(int j) const {
+    uint64_t a = g0;
+#pragma unroll
+    for (int b = 0; b < SB; ++b)
+      if ((j >> b) & 1) a |= gb[b];
+    return a;
+  }
+  __device__ __forceinline__ uint32_t laddr(int j) const {
+    uint32_t a = l0;
+#pragma unroll
+    for (int b = 0; b < SB; ++b)
+      if ((j >> b) & 1) a ^= lb[b];
+    return a;
+  }
+  // complex64: an access holds amplitudes l, l ^ 1 of LDS (h never moves
+  // bit 0 of the index, but may flip it): swap halves when l is odd.
+  __device__ __forceinline__ static void put(ketd::cplx<T>* L, uint32_t l, V v) {
+    if constexpr (EPA == 2) {
+      if (l & 1) v = make_float4(v.z, v.w, v.x, v.y);
+      *reinterpret_cast<V*>(L + (l & ~1u)) = v;
+    } else {
+      *reinterpret_cast<V*>(L + l) = v;
+    }
+  }
+  __device__ __forceinline__ static V get(const ketd::cplx<T>* L, uint32_t l) {
+    if constexpr (EPA == 2) {
+      V v = *reinterpret_cast<const V*>(L + (l & ~1u));
+      if (l & 1) v = make_float4(v.z, v.w, v.x, v.y);
+      return v;
+    } else {
+      return *reinterpret_cast<const V*>(L + l);
+    }
+  }
+  __device__ __forceinline__ void fetch(const ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                        V (&v)[S]) const {
+#pragma unroll
+    for (int j = 0; j < S; ++j) v[j] = *reinterpret_cast<const V*>(s + (fixed | gaddr(j)));
+  }
+  __device__ __forceinline__ void stash(ketd::cplx<T>* L, const V (&v)[S]) const {
+#pragma unroll
+    for (int j = 0; j < S; ++j) put(L, laddr(j), v[j]);
+  }
+  __device__ __forceinline__ void load(const ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                       ketd::cplx<T>* L) const {
+    V v[S];
+    fetch(s, fixed, v);
+    stash(L, v);
+  }
+  __device__ __forceinline__ void store(ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                        const ketd::cplx<T>* L) const {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      *reinterpret_cast<V*>(s + (fixed | gaddr(j))) = get(L, laddr(j));
+  }
+};
+
+// Tile-index offset of element e of a group: bits of e deposited into Q.
+template <int K>
+__device__ __forceinline__ uint32_t qoff(int e, const uint32_t (&qbit)[K]) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int b = 0; b < K; ++b)
+    if ((e >> b) & 1) o |= qbit[b];
+  return o;
+}
+
+template <int K>
+__device__ __forceinline__ void q_bits(const TileMap& m, uint32_t (&qbit)[K]) {
+  uint32_t r = m.qloc;
+#pragma unroll
+  for (int b = 0; b < K; ++b) {
+    qbit[b] = r & (~r + 1);
+    r &= r - 1;
+  }
+}
+
+// Vector tiles: NT = 2^(TB-K) threads, one group each; the matrix and the
+// phase terms are read with wave-uniform addresses (scalar loads).
+template <class T, int K, int TB, bool PH>
+__global__ __launch_bounds__(1 << (TB - K)) void k_block_tile(ketd::cplx<T>* __restrict__ s,
+                                                               uint64_t n_tiles,
+                                                               const unsigned char* __restrict__ rec) {
+  constexpr int D = 1 << K;
+  constexpr int NT = 1 << (TB - K);
+  constexpr int SWB = sizeof(T) == 8 ? 3 : 4;
+  using IO = TileIO<T, TB, NT>;
+  __shared__ ketd::cplx<T> L[1 << TB];
+  const BlockRecord& h = *reinterpret_cast<const BlockRecord*>(rec);
+  const auto* mat = reinterpret_cast<const ketd::cplx<T>*>(rec + h.mat_offset);
+  const auto* tm = reinterpret_cast<const uint64_t*>(rec + h.terms_offset);
+  const auto* th = reinterpret_cast<const double*>(tm + h.n_terms);
+  const TileMap mp = tile_map<K, TB, SWB>(h);
+  uint32_t qbit[K];
+  q_bits<K>(mp, qbit);
+  IO io;
+  io.init(mp, threadIdx.x);
+  const uint32_t tg = (uint32_t)deposit(threadIdx.x, mp.nloc);
+  const uint32_t lg = swz(tg, mp);
+  const uint64_t gg = deposit(tg, mp.pmask);
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t fixed = deposit_outside(tile, mp.pmask);
+    io.load(s, fixed, L);
+    __syncthreads();
+    ketd::cplx<T> in[D];
+#pragma unroll
+    for (int e = 0; e < D; ++e) in[e] = L[lg ^ qoff<K>(e, qbit)];
+    PhaseFactors<T, K> ph;
+    if constexpr (PH) {
+      double C, A[K];
+      group_angles<K>(fixed | gg, h, tm, th, C, A);
+      ph.set(C, A);
+    }
+    // Each thread owns its group's LDS slots: it has read all of them
+    // before it writes any, so no barrier is needed in between.
+#pragma unroll 1
+    for (int r = 0; r < D; ++r) {
+      T re = 0, im = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const ketd::cplx<T> m = mat[r * D + c];
+        re = fma(m.x, in[c].x, re);
+        re = fma(-m.y, in[c].y, re);
+        im = fma(m.x, in[c].y, im);
+        im = fma(m.y, in[c].x, im);
+      }
+      ketd::cplx<T> o = ketd::mk<T>(re, im);
+      if constexpr (PH) o = ketd::cmul<T>(o, ph.at(r));
+      L[lg ^ qoff<K>(r, qbit)] = o;
+    }
+    __syncthreads();
+    io.store(s, fixed, L);
+    __syncthreads();
+  }
+}
+
+// complex64 MFMA tiles: 256 threads, TB = 12; each wave applies the block
+// to TPW tiles of 16 groups with the operand layout of k_block_mfma.
+template <int K, bool PH>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 ? 3 : 1)))
+void k_block_tile_mfma(float2* __restrict__ s,
+                                                               uint64_t n_tiles,
+                                                               const unsigned char* __restrict__ rec) {
+  constexpr int TB = 12;
+  constexpr int D = 1 << K;
+  constexpr int RB = 2 * D / 16;
+  constexpr int KS = 2 * D / 4;
+  constexpr int NE = D / 4;
+  constexpr int TPW = (1 << (TB - K)) / 16 / (kThreads / 64);
+  using IO = TileIO<float, TB, kThreads>;
+  constexpr int G = 1 << (TB - K);  // groups per tile
+  __shared__ float2 L[1 << TB];
+  __shared__ uint64_t tm[PH ? kMaxTerms : 1];
+  __shared__ double th[PH ? kMaxTerms : 1];
+  __shared__ float2 PT[PH ? (K + 1) * G : 1];
+  const BlockRecord& h = *reinterpret_cast<const BlockRecord*>(rec);
+  if constexpr (PH) {
+    const auto* gm = reinterpret_cast<const uint64_t*>(rec + h.terms_offset);
+    const auto* ga = reinterpret_cast<const double*>(gm + h.n_terms);
+    for (int i = threadIdx.x; i < h.n_terms; i += kThreads) {
+      tm[i] = gm[i];
+      th[i] = ga[i];
+    }
+  }
+  const TileMap mp = tile_map<K, TB, 4>(h);
+  uint32_t qbit[K];
+  q_bits<K>(mp, qbit);
+  IO io;
+  io.init(mp, threadIdx.x);
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15;
+  const int q = lane >> 4;
+  const int wave = threadIdx.x >> 6;
+  const auto* M = reinterpret_cast<const float2*>(rec + h.mat_offset);
+  float a[RB][KS];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int row = rb * 16 + col, c = ks * 4 + q;
+      const float2 m = M[(row % D) * D + (c % D)];
+      const bool im_r = row >= D, im_c = c >= D;
+      a[rb][ks] = !im_r ? (!im_c ? m.x : -m.y) : (!im_c ? m.y : m.x);
+    }
+  uint32_t lg[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u)
+    lg[u] = swz((uint32_t)deposit((wave * TPW + u) * 16 + col, mp.nloc), mp);
+  // Lane (col, q) reads elements q + 4i and writes elements with bits 2..3
+  // equal to q: bits 0..1 and 2..3 of the element index, deposited.
+  const uint32_t oq_in = qoff<K>(q, qbit), oq_out = qoff<K>(4 * q, qbit);
+  // Phase table of the tile: exp(i * angle of range r) of group g at
+  // PT[r * G + g] (r = 0: E, r = j + 1: F_j). While the tile's loads are in
+  // flight, thread t fills ranges r = t / G (mod kThreads / G) of group
+  // t % G, summing the terms in the four interleaved partial sums of
+  // k_block_mfma in its shuffle order, so both kernels get the same angles.
+  constexpr int SPLIT = kThreads / G;
+  // part is uniform in a wave (G is a multiple of 64); as a scalar it keeps
+  // the range bounds in scalar loads, which do not wait on the tile's loads.
+  const int pg = threadIdx.x % G;
+  const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / G);
+  const uint64_t my_gg = PH ? deposit(deposit(pg, mp.nloc), mp.pmask) : 0;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t fixed = deposit_outside(tile, mp.pmask);
+    typename IO::V v[IO::S];
+    io.fetch(s, fixed, v);
+    if constexpr (PH) {
+      const uint64_t x = fixed | my_gg;
+      for (int r = part; r <= K; r += SPLIT) {
+        const double p01 = range_angle(x, h, tm, th, r, 0, 4) + range_angle(x, h, tm, th, r, 1, 4);
+        const double p23 = range_angle(x, h, tm, th, r, 2, 4) + range_angle(x, h, tm, th, r, 3, 4);
+        PT[r * G + pg] = expi<float>(p01 + p23);
+      }
+    }
+    io.stash(L, v);
+    __syncthreads();
+    floatx4 acc[TPW][RB];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      float2 in[NE];
+#pragma unroll
+      for (int i = 0; i < NE; ++i) in[i] = L[lg[u] ^ oq_in ^ qoff<K>(4 * i, qbit)];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) acc[u][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const float b = ks < NE ? in[ks].x : in[ks - NE].y;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          acc[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[u][rb], 0, 0, 0);
+      }
+    }
+    // The four lanes of a group read and write overlapping slots.
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      // Output (rb, v) of the lane gets Gq * F0^v0 * F1^v1 * prod_{j>=4}
+      // Fj^(bit j-4 of rb), Gq = E * F2^q0 * F3^q1 (bits 2, 3 of every
+      // element the lane writes are q).
+      const int g = (wave * TPW + u) * 16 + col;
+      float2 Gq = make_float2(1.f, 0.f), F[K];
+      if constexpr (PH) {
+        Gq = PT[g];
+#pragma unroll
+        for (int j = 0; j < K; ++j) F[j] = PT[(j + 1) * G + g];
+        if (q & 1) Gq = ketd::cmul<float>(Gq, F[2]);
+        if (q & 2) Gq = ketd::cmul<float>(Gq, F[3]);
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB / 2; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float2 o = make_float2(acc[u][rb][v], acc[u][rb + RB / 2][v]);
+          if constexpr (PH) {
+            float2 p = Gq;
+            if (v & 1) p = ketd::cmul<float>(p, F[0]);
+            if (v & 2) p = ketd::cmul<float>(p, F[1]);
+#pragma unroll
+            for (int j = 4; j < K; ++j)
+              if ((rb >> (j - 4)) & 1) p = ketd::cmul<float>(p, F[j]);
+            o = ketd::cmul<float>(o, p);
+          }
+          L[lg[u] ^ oq_out ^ qoff<K>(rb * 16 + v, qbit)] = o;
+        }
+    }
+    __syncthreads();
+    io.store(s, fixed, L);
+    __syncthreads();
+  }
+}
+
 inline unsigned capped_grid(uint64_t items, uint64_t cap) {
   uint64_t g = (items + kThreads - 1) / kThreads;
   if (g > cap) g = cap;
@@ -295,7 +694,7 @@ inline unsigned capped_grid(uint64_t items, uint64_t cap) {
 template <class T, bool PH>
 hipError_t block_vec(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
   const Tuning t = get_tuning();
-  const unsigned g = capped_grid(groups, 256 * (uint64_t)(t.block_bpc > 0 ? 2 * t.block_bpc : 16));
+  const unsigned g = capped_grid(groups, 256 * (uint64_t)(t.block_bpc > 0 ? 2 * t.block_bpc : 64));
   auto* p = static_cast<ketd::cplx<T>*>(s);
   switch (k) {
     case 1: k_block_vec<T, 1, PH><<<g, kThreads, 0, st>>>(p, groups, rec); break;
@@ -317,7 +716,7 @@ hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const uns
   const int tpi = t.block_tiles == 2 ? 2 : 1;
   const uint64_t tiles = groups / 16;
   uint64_t blocks = (tiles + 4 * tpi - 1) / (4 * tpi);
-  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 8);
+  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
   if (blocks > cap) blocks = cap;
   if (blocks == 0) blocks = 1;
   auto* p = static_cast<float2*>(s);
@@ -329,6 +728,35 @@ hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const uns
     if (tpi == 2) k_block_mfma<5, PH, 2><<<b, kThreads, 0, st>>>(p, groups, rec);
     else k_block_mfma<5, PH, 1><<<b, kThreads, 0, st>>>(p, groups, rec);
   }
+  return hipGetLastError();
+}
+
+template <class T, bool PH>
+hipError_t block_tile(hipStream_t st, void* s, int n, int k, const unsigned char* rec) {
+  constexpr int TB = 11;
+  const Tuning t = get_tuning();
+  const uint64_t tiles = uint64_t{1} << (n - TB);
+  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
+  const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
+  auto* p = static_cast<ketd::cplx<T>*>(s);
+  switch (k) {
+    case 3: k_block_tile<T, 3, TB, PH><<<g, 1 << (TB - 3), 0, st>>>(p, tiles, rec); break;
+    case 4: k_block_tile<T, 4, TB, PH><<<g, 1 << (TB - 4), 0, st>>>(p, tiles, rec); break;
+    case 5: k_block_tile<T, 5, TB, PH><<<g, 1 << (TB - 5), 0, st>>>(p, tiles, rec); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <bool PH>
+hipError_t block_tile_mfma(hipStream_t st, void* s, int n, int k, const unsigned char* rec) {
+  const Tuning t = get_tuning();
+  const uint64_t tiles = uint64_t{1} << (n - 12);
+  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
+  const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
+  auto* p = static_cast<float2*>(s);
+  if (k == 4) k_block_tile_mfma<4, PH><<<g, kThreads, 0, st>>>(p, tiles, rec);
+  else k_block_tile_mfma<5, PH><<<g, kThreads, 0, st>>>(p, tiles, rec);
   return hipGetLastError();
 }
 
@@ -371,13 +799,25 @@ hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, i
 }
 
 hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
-                        int n_terms, const void* rec_dev) {
+                        int n_terms, const void* rec_dev, int lowest) {
   if (k < 1 || k > 6 || k > n || n_terms < 0 || n_terms > kMaxTerms) return hipErrorInvalidValue;
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
   const uint64_t groups = (uint64_t{1} << n) >> k;
   const bool ph = n_terms > 0;
+  const Tuning t = get_tuning();
+  // MFMA blocks: LDS tiles at any position (block_stage > 0); vector
+  // blocks: only when their lowest qubit is below block_stage, since the
+  // direct kernel's group loads coalesce once the low qubits are outside.
+  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= 12)
+    return ph ? block_tile_mfma<true>(st, s, n, k, rec) : block_tile_mfma<false>(st, s, n, k, rec);
   if (precision == 1 && (k == 4 || k == 5) && groups >= 16)
     return ph ? block_mfma<true>(st, s, groups, k, rec) : block_mfma<false>(st, s, groups, k, rec);
+  const bool staged = lowest >= 0 && lowest < t.block_stage && k >= 3 && k <= 5;
+  if (staged && n >= 11) {
+    if (precision == 0)
+      return ph ? block_tile<double, true>(st, s, n, k, rec) : block_tile<double, false>(st, s, n, k, rec);
+    return ph ? block_tile<float, true>(st, s, n, k, rec) : block_tile<float, false>(st, s, n, k, rec);
+  }
   if (precision == 0)
     return ph ? block_vec<double, true>(st, s, groups, k, rec)
               : block_vec<double, false>(st, s, groups, k, rec);

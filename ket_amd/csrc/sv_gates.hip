@@ -201,7 +201,7 @@ cplx<T> to_c(const double* m, int i) {
 }
 
 // Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json).
-Tuning g_tuning{2, 0, 0, 1, 8};
+Tuning g_tuning{2, 0, 0, 1, 32, 3};
 
 inline unsigned tuned_grid(uint64_t items, int per_thread) {
   const uint64_t per_block = (uint64_t)kThreads * per_thread;

@@ -141,20 +141,28 @@ def device_info() -> dict:
 class Tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("block_tiles", ctypes.c_int32),
-                ("block_bpc", ctypes.c_int32)]
+                ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32)]
 
 
-def set_tuning(items_per_thread=2, nontemporal=0, blocks_per_cu=0, block_tiles=0, block_bpc=0):
-    t = Tuning(items_per_thread, nontemporal, blocks_per_cu, block_tiles, block_bpc)
+TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tiles", "block_bpc",
+                 "block_stage")
+
+
+def set_tuning(**changes):
+    """Sets the named ket_tuning fields; the others keep their current values."""
+    unknown = set(changes) - set(TUNING_FIELDS)
+    if unknown:
+        raise TypeError(f"set_tuning: unknown fields {sorted(unknown)}")
+    cur = get_tuning()
+    cur.update(changes)
+    t = Tuning(*(int(cur[f]) for f in TUNING_FIELDS))
     check(lib().ket_set_tuning(ctypes.byref(t)))
 
 
 def get_tuning() -> dict:
     t = Tuning()
     check(lib().ket_get_tuning(ctypes.byref(t)))
-    return {"items_per_thread": t.items_per_thread, "nontemporal": t.nontemporal,
-            "blocks_per_cu": t.blocks_per_cu, "block_tiles": t.block_tiles,
-            "block_bpc": t.block_bpc}
+    return {f: getattr(t, f) for f in TUNING_FIELDS}
 
 
 def calibrate_copy(nbytes: int = 4 << 30, reps: int = 10, nontemporal: int = 0) -> float:
@@ -262,7 +270,7 @@ class StateVector:
         check(lib().ket_sv_apply_matrix(self._h, len(q), _p(q), _p(m)))
 
     def apply_records(self, rec: np.ndarray, fusion: int = FUSE_REFERENCE,
-                      block_k: int = 5) -> dict:
+                      block_k: int = 0) -> dict:
         r = np.ascontiguousarray(rec, dtype=GATE_DTYPE)
         st = RunStats()
         check(lib().ket_sv_apply_circuit(self._h, _p(r), len(r), int(fusion), int(block_k),
@@ -270,10 +278,10 @@ class StateVector:
         self.last_stats = st.as_dict()
         return self.last_stats
 
-    def apply_circuit(self, circuit, fusion: int = FUSE_REFERENCE, block_k: int = 5) -> dict:
+    def apply_circuit(self, circuit, fusion: int = FUSE_REFERENCE, block_k: int = 0) -> dict:
         return self.apply_records(circuit.records(), fusion, block_k)
 
-    def profile_circuit(self, circuit, fusion: int = FUSE_NONE, block_k: int = 5):
+    def profile_circuit(self, circuit, fusion: int = FUSE_NONE, block_k: int = 0):
         """Runs the circuit, timing each launch; returns a list of dicts."""
         r = np.ascontiguousarray(circuit.records(), dtype=GATE_DTYPE)
         cap = 4 * len(r) + 16

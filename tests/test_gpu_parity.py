@@ -106,6 +106,44 @@ def test_blocks_complex128(cname, k):
     assert 1 - fid(got, want) <= 1e-12
 
 
+STAGED = {
+    "qft11": lambda: W.qft_u(11, seed=9),
+    "brick11": lambda: W.brickwork(11, 8, seed=10),
+    "mixed12": CIRCUITS["mixed12"],
+    "qft12": CIRCUITS["qft12"],
+    "bwm13": CIRCUITS["bwm13"],
+    "diag12": CIRCUITS["diag12"],
+    "qft14": CIRCUITS["qft14"],
+    "brick14": CIRCUITS["brick14"],
+}
+
+
+@pytest.fixture
+def restore_tuning():
+    t = N.get_tuning()
+    yield
+    N.set_tuning(**t)
+
+
+@pytest.mark.parametrize("cname", sorted(STAGED))
+@pytest.mark.parametrize("precision,k", [(N.C128, 3), (N.C128, 4), (N.C128, 5),
+                                         (N.C64, 3), (N.C64, 4), (N.C64, 5)])
+def test_blocks_lds_tiles_equal_direct(cname, precision, k, restore_tuning):
+    """Blocks moved through LDS tiles (every block, block_stage 64) give the
+    same bits as the direct kernels (block_stage 0): same products in the
+    same order, only the memory path differs."""
+    c = STAGED[cname]()
+    t = N.get_tuning()
+    N.set_tuning(**{**t, "block_stage": 0})
+    direct = gpu_run(c, N.FUSE_BLOCKS, precision, k)
+    N.set_tuning(**{**t, "block_stage": 64})
+    staged = gpu_run(c, N.FUSE_BLOCKS, precision, k)
+    assert np.array_equal(staged, direct), np.abs(staged - direct).max()
+    want = O.run(c)
+    tol = 1e-12 if precision == N.C128 else 1e-6
+    assert 1 - fid(staged, want) <= tol
+
+
 @pytest.mark.parametrize("cname", sorted(CIRCUITS))
 @pytest.mark.parametrize("fusion,k", [(N.FUSE_NONE, 5), (N.FUSE_REFERENCE, 5),
                                       (N.FUSE_BLOCKS, 4), (N.FUSE_BLOCKS, 5)])
