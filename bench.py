@@ -356,7 +356,7 @@ def main():
     out = {"metric": METRIC}
     if args.workload == "brick33":
         return bench_brick33(args, circ, desc)
-    copy_gbps = N.calibrate_copy(4 << 30, 5, 0)
+    copy_gbps = max(N.calibrate_copy(4 << 30, 5, nt) for nt in (0, 1))
     log(f"[bench] copy kernel (4 GiB, read+write): {copy_gbps:.0f} GB/s")
     plain = N.StateVector(n, N.C128)
     log(f"[bench] plain complex128: warmup {args.warmup}, steps {args.steps}")

@@ -255,7 +255,7 @@ int ket_mlp_use_for_planning(ket_mlp* m);
  *      process-wide; no counterpart in the reference, whose CPU loops have
  *      no launch shape) ---- */
 typedef struct ket_tuning {
-  int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 2, 4, 8 */
+  int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 1, 2, 4, 8 */
   int32_t nontemporal;      /* 1: streaming (nontemporal) loads and stores */
   int32_t blocks_per_cu;    /* 0: one block per 256*items items; else cap the
                                grid at 256*blocks_per_cu blocks (grid-stride) */

@@ -25,7 +25,7 @@ from .gates import Circuit
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-SPACE = {"items_per_thread": [2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32]}
+SPACE = {"items_per_thread": [1, 2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32]}
 
 
 def probe_circuit(n: int) -> Circuit:
@@ -46,7 +46,7 @@ def probe_circuit(n: int) -> Circuit:
     return c
 
 
-BLOCK_SPACE = {"block_tiles": [1, 2], "block_bpc": [4, 8, 16, 32]}
+BLOCK_SPACE = {"block_stage": [0, 3, 64], "block_bpc": [8, 16, 32]}
 
 
 def block_probe(n: int, k: int) -> Circuit:
@@ -68,11 +68,11 @@ def block_probe(n: int, k: int) -> Circuit:
 def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -> dict:
     k = 5 if precision == N.C64 else 4
     circ = block_probe(n, k)
-    base = N.get_tuning()
     results = []
     with N.StateVector(n, precision) as sv:
-        for bt, bpc in itertools.product(*BLOCK_SPACE.values()):
-            N.set_tuning(block_tiles=bt, block_bpc=bpc)
+        for point in itertools.product(*BLOCK_SPACE.values()):
+            cfg = dict(zip(BLOCK_SPACE, point))
+            N.set_tuning(**cfg)
             sv.profile_circuit(circ, N.FUSE_BLOCKS_ALL, k)
             ms = []
             for _ in range(reps):
@@ -80,16 +80,16 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
                         if r["kind"] == "block"]
                 ms.append(sum(r["ms"] for r in recs))
             bytes_ = sum(r["alg_bytes"] for r in recs)
-            m = {"block_tiles": bt, "block_bpc": bpc, "ms": float(np.median(ms)),
+            m = {**cfg, "ms": float(np.median(ms)),
                  "GBps": bytes_ / (float(np.median(ms)) / 1e3) / 1e9,
                  "per_block_GBps": {f"q{(r['qubit_mask'] & -r['qubit_mask']).bit_length() - 1}":
                                     round(r["alg_bytes"] / (r["ms"] / 1e3) / 1e9)
                                     for r in recs}}
             results.append(m)
-            log(f"[tune] blocks k={k} tiles={bt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
+            log(f"[tune] blocks k={k} {cfg}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 f"by lowest qubit {m['per_block_GBps']}")
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(block_tiles=best["block_tiles"], block_bpc=best["block_bpc"])
+    N.set_tuning(**{kk: best[kk] for kk in BLOCK_SPACE})
     return {"k": k, "best": {kk: best[kk] for kk in BLOCK_SPACE}, "best_GBps": best["GBps"],
             "all": results}
 

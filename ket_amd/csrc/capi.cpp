@@ -692,8 +692,9 @@ int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity, double* max_abs_diff
 
 int ket_set_tuning(const ket_tuning* t) {
   if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
-  if (t->items_per_thread != 2 && t->items_per_thread != 4 && t->items_per_thread != 8)
-    return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 2, 4 or 8");
+  if (t->items_per_thread != 1 && t->items_per_thread != 2 && t->items_per_thread != 4 &&
+      t->items_per_thread != 8)
+    return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 1, 2, 4 or 8");
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
     return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
   if (t->block_tiles < 0 || t->block_tiles > 2 || t->block_bpc < 0 || t->block_bpc > 64)

@@ -200,8 +200,10 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
-// Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json).
-Tuning g_tuning{2, 0, 0, 1, 32, 3};
+// Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json):
+// one pair per thread over a full grid; fused blocks with 32 workgroups per CU,
+// LDS tiles for complex64 MFMA blocks and for blocks on qubits 0..2.
+Tuning g_tuning{1, 0, 0, 1, 32, 3};
 
 inline unsigned tuned_grid(uint64_t items, int per_thread) {
   const uint64_t per_block = (uint64_t)kThreads * per_thread;
@@ -210,7 +212,7 @@ inline unsigned tuned_grid(uint64_t items, int per_thread) {
     const uint64_t cap = (uint64_t)256 * g_tuning.blocks_per_cu;
     if (g > cap) g = cap;
   }
-  if (g > (uint64_t{1} << 20)) g = uint64_t{1} << 20;
+  if (g > (uint64_t{1} << 22)) g = uint64_t{1} << 22;  // 2^30 work-items
   if (g == 0) g = 1;
   return (unsigned)g;
 }
@@ -226,6 +228,7 @@ template <class T, int OP>
 void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
   const bool nt = g_tuning.nontemporal != 0;
   switch (g_tuning.items_per_thread) {
+    case 1: nt ? pair_go<T, OP, 1, 1>(st, p, sp, m) : pair_go<T, OP, 1, 0>(st, p, sp, m); break;
     case 2: nt ? pair_go<T, OP, 2, 1>(st, p, sp, m) : pair_go<T, OP, 2, 0>(st, p, sp, m); break;
     case 8: nt ? pair_go<T, OP, 8, 1>(st, p, sp, m) : pair_go<T, OP, 8, 0>(st, p, sp, m); break;
     default: nt ? pair_go<T, OP, 4, 1>(st, p, sp, m) : pair_go<T, OP, 4, 0>(st, p, sp, m); break;
@@ -254,6 +257,7 @@ hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
   auto* p = static_cast<cplx<T>*>(s);
   const bool nt = g_tuning.nontemporal != 0;
   switch (g_tuning.items_per_thread) {
+    case 1: nt ? phase_go<T, 1, 1>(st, p, sp, ph) : phase_go<T, 1, 0>(st, p, sp, ph); break;
     case 2: nt ? phase_go<T, 2, 1>(st, p, sp, ph) : phase_go<T, 2, 0>(st, p, sp, ph); break;
     case 8: nt ? phase_go<T, 8, 1>(st, p, sp, ph) : phase_go<T, 8, 0>(st, p, sp, ph); break;
     default: nt ? phase_go<T, 4, 1>(st, p, sp, ph) : phase_go<T, 4, 0>(st, p, sp, ph); break;
@@ -261,27 +265,15 @@ hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
   return hipGetLastError();
 }
 
-// Bandwidth calibration: a plain float4 copy with the per-gate passes'
-// launch shape (two 16-byte items per thread, blocks over the whole range,
-// grid-stride beyond 2^20 blocks).
+// Bandwidth calibration: a plain float4 copy, one 16-byte item per thread
+// over the whole range (grid-stride beyond 2^22 blocks): the fastest shape
+// among copies with 1, 2, 4, 8 items per thread and capped grids on MI355X.
 template <int NT>
 __global__ __launch_bounds__(kThreads) void k_copy(const float4* __restrict__ a,
                                                     float4* __restrict__ b, uint64_t n) {
-  const uint64_t per_block = (uint64_t)kThreads * 2;
-  for (uint64_t first = (uint64_t)blockIdx.x * per_block + threadIdx.x;
-       first - threadIdx.x < n; first += (uint64_t)gridDim.x * per_block) {
-    float4 v[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint64_t i = first + (uint64_t)j * kThreads;
-      if (i < n) v[j] = ld<NT>(a + i);
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint64_t i = first + (uint64_t)j * kThreads;
-      if (i < n) st<NT>(b + i, v[j]);
-    }
-  }
+  for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kThreads)
+    st<NT>(b + i, ld<NT>(a + i));
 }
 
 }  // namespace
@@ -317,8 +309,8 @@ hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes, i
                        int blocks) {
   const uint64_t n = bytes / sizeof(float4);
   if (blocks <= 0) {
-    uint64_t g = (n + 511) / 512;
-    if (g > (uint64_t{1} << 20)) g = uint64_t{1} << 20;
+    uint64_t g = (n + kThreads - 1) / kThreads;
+    if (g > (uint64_t{1} << 22)) g = uint64_t{1} << 22;
     blocks = (int)(g ? g : 1);
   }
   if (nt)
