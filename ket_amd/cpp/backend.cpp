@@ -1,5 +1,7 @@
 // Routing and the state-vector Simulation on the GPU
 // (reference src/backends/backend.cpp:16-80, statevector.cpp:26-58).
+// Clifford circuits under Method::Auto or Method::Stabilizer go to the
+// tableau backend (stabilizer.cpp), as in ket.
 //
 // StateVectorSimulation keeps the evolved state resident on the device and
 // answers sample()/expval() with device reductions; only the drawn uniform
@@ -161,10 +163,7 @@ std::unique_ptr<Simulation> simulate(const Circuit& circuit, Method method) {
         throw std::invalid_argument(
             "stabilizer backend requires a Clifford circuit (it contains a "
             "non-Clifford gate); use Method::Auto or Method::StateVector");
-      // This build has no tableau engine: Clifford circuits are evolved as
-      // dense states on the GPU, which bounds them by device memory.
-      return std::make_unique<GpuStateVectorSimulation>(evolved(circuit), circuit,
-                                                        "state vector");
+      return detail::make_stabilizer_simulation(circuit);
     case Method::Auto:
       break;
   }

@@ -59,4 +59,7 @@ std::shared_ptr<DeviceState> evolve(const Circuit& c,
 // One shot's uniform, drawn as ket's measure() draws it.
 double draw_uniform(std::mt19937& rng);
 
+// The tableau backend for Clifford circuits (stabilizer.cpp).
+std::unique_ptr<Simulation> make_stabilizer_simulation(const Circuit& circuit);
+
 }  // namespace ket::detail

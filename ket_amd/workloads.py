@@ -154,3 +154,30 @@ def diag_mix(n: int, depth: int, seed: int) -> Circuit:
         c.cx(a, b)
         c.h(int(rng.integers(n)))
     return c
+
+
+CLIFFORD_1Q = ("h", "s", "sdg", "x", "y", "z")
+CLIFFORD_2Q = ("cx", "cy", "cz", "swap")
+
+
+def random_clifford(n: int, depth: int, seed: int, measures: int = 0) -> Circuit:
+    """Random circuit over ket's Clifford set (H, S, Sdg, X, Y, Z, CX, CY, CZ,
+    SWAP), with `measures` measurements of random qubits into random clbits
+    at random positions (mid-circuit and repeated qubits included)."""
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    slots = sorted(rng.choice(depth + 1, size=measures, replace=True)) if measures else []
+    k = 0
+    for d in range(depth):
+        while k < len(slots) and slots[k] == d:
+            c.measure(int(rng.integers(n)), int(rng.integers(max(1, min(n, 64)))))
+            k += 1
+        if n > 1 and rng.random() < 0.4:
+            a, b = rng.choice(n, 2, replace=False)
+            getattr(c, CLIFFORD_2Q[rng.integers(len(CLIFFORD_2Q))])(int(a), int(b))
+        else:
+            getattr(c, CLIFFORD_1Q[rng.integers(len(CLIFFORD_1Q))])(int(rng.integers(n)))
+    while k < len(slots):
+        c.measure(int(rng.integers(n)), int(rng.integers(max(1, min(n, 64)))))
+        k += 1
+    return c

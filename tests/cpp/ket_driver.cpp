@@ -19,6 +19,8 @@
 //   sample-calls <circ> <seed> <shots> <out.bin>
 //       the reference's per-call ket::sample(circuit, rng) loop (re-evolves)
 //   expval <circ> <pauli>...           one double per line
+//   stab-sample <circ> <seed> <shots> <out.bin>
+//   stab-expval <circ> <pauli>...      the same with Method::Stabilizer
 //   uniforms <seed> <count> <out.bin>  uniform_real_distribution<double>(0,1)
 #include <chrono>
 #include <cstdint>
@@ -122,8 +124,8 @@ void write_u64(const std::vector<std::uint64_t>& v, const std::string& path) {
 }
 
 int usage() {
-  std::cerr << "usage: ket_driver run|time|sample|sample-calls|expval|uniforms"
-               " ...\n";
+  std::cerr << "usage: ket_driver run|time|sample|sample-calls|expval|stab-sample|"
+               "stab-expval|uniforms ...\n";
   return 2;
 }
 
@@ -157,14 +159,15 @@ int main(int argc, char** argv) {
       std::cout << "]}\n";
       return 0;
     }
-    if ((mode == "sample" || mode == "sample-calls") && argc >= 6) {
+    if ((mode == "sample" || mode == "sample-calls" || mode == "stab-sample") && argc >= 6) {
       const ket::Circuit c = parse_circuit(argv[2]);
       std::mt19937 rng{static_cast<std::mt19937::result_type>(std::stoul(argv[3]))};
       const std::size_t shots = std::stoul(argv[4]);
       std::vector<std::uint64_t> out;
       out.reserve(shots);
-      if (mode == "sample") {
-        const auto sim = ket::simulate(c, ket::Method::StateVector);
+      if (mode == "sample" || mode == "stab-sample") {
+        const auto sim = ket::simulate(
+            c, mode == "sample" ? ket::Method::StateVector : ket::Method::Stabilizer);
         for (std::size_t i = 0; i < shots; ++i) out.push_back(pack(sim->sample(rng)));
       } else {
         for (std::size_t i = 0; i < shots; ++i)
@@ -173,9 +176,10 @@ int main(int argc, char** argv) {
       write_u64(out, argv[5]);
       return 0;
     }
-    if (mode == "expval" && argc >= 4) {
+    if ((mode == "expval" || mode == "stab-expval") && argc >= 4) {
       const ket::Circuit c = parse_circuit(argv[2]);
-      const auto sim = ket::simulate(c, ket::Method::StateVector);
+      const auto sim = ket::simulate(
+          c, mode == "expval" ? ket::Method::StateVector : ket::Method::Stabilizer);
       std::printf("%s\n", std::string(sim->backend()).c_str());
       for (int i = 3; i < argc; ++i) std::printf("%.17g\n", sim->expval(argv[i]));
       return 0;

@@ -367,6 +367,83 @@ TEST(routing) {
   CHECK(!ket::is_clifford(outer));
   // Clifford expectation values agree with the dense state.
   NEAR(ket::expval(cl, std::string("IZZ")), ket::expval(ket::run(cl), "IZZ"), 1e-9);
+  CHECK(ket::simulate(cl)->backend() == "stabilizer");
+  CHECK(ket::simulate(cl, ket::Method::StateVector)->backend() == "state vector");
+}
+
+// ---- the tableau backend (reference test_stabilizer.cpp:84-214) ----
+namespace {
+ket::Circuit random_clifford(std::size_t n, int depth, std::mt19937& rng) {
+  ket::Circuit c(n);
+  std::uniform_int_distribution<int> pick(0, 9);
+  std::uniform_int_distribution<std::size_t> qubit(0, n - 1);
+  for (int d = 0; d < depth; ++d) {
+    const std::size_t a = qubit(rng);
+    std::size_t b = qubit(rng);
+    while (n > 1 && b == a) b = qubit(rng);
+    const int g = pick(rng);
+    if (g >= 6 && n < 2) continue;
+    switch (g) {
+      case 0: c.h(a); break;
+      case 1: c.s(a); break;
+      case 2: c.sdg(a); break;
+      case 3: c.x(a); break;
+      case 4: c.y(a); break;
+      case 5: c.z(a); break;
+      case 6: c.cx(a, b); break;
+      case 7: c.cy(a, b); break;
+      case 8: c.cz(a, b); break;
+      default: c.swap(a, b); break;
+    }
+  }
+  return c;
+}
+}  // namespace
+
+TEST(stabilizer_expval_equals_dense_on_every_pauli) {
+  std::mt19937 rng{1234};
+  const char letters[] = {'I', 'X', 'Y', 'Z'};
+  for (std::size_t n = 1; n <= 4; ++n)
+    for (int trial = 0; trial < 10; ++trial) {
+      const ket::Circuit c = random_clifford(n, 25, rng);
+      const ket::State dense = ket::run(c);
+      const auto stab = ket::simulate(c, ket::Method::Stabilizer);
+      std::size_t total = 1;
+      for (std::size_t i = 0; i < n; ++i) total *= 4;
+      for (std::size_t k = 0; k < total; ++k) {
+        std::string p(n, 'I');
+        for (std::size_t i = 0, v = k; i < n; ++i, v /= 4) p[n - 1 - i] = letters[v % 4];
+        NEAR(stab->expval(p), ket::expval(dense, p), 1e-9);
+        NEAR(ket::expval(c, p), ket::expval(dense, p), 1e-9);  // Auto -> tableau
+      }
+    }
+}
+
+TEST(stabilizer_ghz_and_deterministic_shots) {
+  ket::Circuit g(3);
+  g.h(0); g.cx(0, 1); g.cx(1, 2); g.measure_all();
+  std::mt19937 rng{7};
+  const auto sim = ket::simulate(g, ket::Method::Stabilizer);
+  CHECK(sim->backend() == "stabilizer");
+  int zeros = 0, ones = 0;
+  for (int shot = 0; shot < 400; ++shot) {
+    const std::vector<int> r = sim->sample(rng);
+    CHECK(r.size() == 3 && r[0] == r[1] && r[1] == r[2]);
+    (r[0] ? ones : zeros)++;
+  }
+  CHECK(zeros > 0 && ones > 0);
+  ket::Circuit d(2);
+  d.x(0); d.h(1); d.h(1); d.measure_all();
+  const auto det = ket::simulate(d, ket::Method::Stabilizer);
+  std::mt19937 r0{0};
+  for (int shot = 0; shot < 50; ++shot) {
+    const std::vector<int> r = det->sample(r0);
+    CHECK(r[0] == 1 && r[1] == 0);
+  }
+  // A deterministic shot draws nothing from the generator.
+  CHECK(r0() == std::mt19937{0}());
+  THROWS(std::invalid_argument, det->expval("Z"));
+  THROWS(std::invalid_argument, det->expval("ZW"));
 }
 
 // ---- stepper (reference test_stepper.cpp) ----
