@@ -69,7 +69,7 @@ struct Launch {
   // Lds: offsets of the arrays in the pool
   std::size_t nq_off = 0, ops_off = 0, opoff_off = 0, z_off = 0, c_off = 0, toff_off = 0;
   int expval_terms = 0;
-  int lowest = -1;             // BlockSingle: the block's lowest qubit
+  int lowest = -1;             // Block, BlockSingle: the block's lowest qubit
 };
 
 std::size_t align(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
@@ -322,6 +322,7 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
       L.kind = Launch::Block;
       L.n = n;
       L.k = p0.ops[op].k;
+      L.lowest = ketplan::lowest_qubit(p0.ops[op]);
       L.n_terms = p0.ops[op].n_terms;
       L.instances = (unsigned)inst.size();
       L.arena_off = base;
@@ -439,7 +440,7 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
           break;
         case Launch::Block:
           e = ketk::launch_block_batched(b->stream, P, arena, L.n, L.k, L.n_terms, pd + L.pool_off,
-                                         L.rec_stride, L.instances);
+                                         L.rec_stride, L.instances, L.lowest);
           break;
         case Launch::BlockSingle:
           e = ketk::launch_block(b->stream, P, arena, L.n, L.k, L.n_terms, pd + L.pool_off,

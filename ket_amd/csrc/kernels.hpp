@@ -79,10 +79,11 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
 
 // The same block pass on `instances` states of n qubits laid out back to
 // back (instance i at s + i * 2^n amplitudes), instance i's record at
-// rec_dev + i * rec_stride bytes (all records share k and n_terms).
+// rec_dev + i * rec_stride bytes (all records share k, n_terms and the
+// block qubits, whose lowest is `lowest`).
 hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, int k,
                                 int n_terms, const void* rec_dev, uint64_t rec_stride,
-                                unsigned instances);
+                                unsigned instances, int lowest);
 
 // ---- batched small instances held entirely in LDS (sv_batch.hip) ----
 // One op of an instance's circuit in LDS: pair and phase passes over the

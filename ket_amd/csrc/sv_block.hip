@@ -493,9 +493,9 @@ __device__ __forceinline__ void q_bits(const TileMap& m, uint32_t (&qbit)[K]) {
 // Vector tiles: NT = 2^(TB-K) threads, one group each; the matrix and the
 // phase terms are read with wave-uniform addresses (scalar loads).
 template <class T, int K, int TB, bool PH>
-__global__ __launch_bounds__(1 << (TB - K)) void k_block_tile(ketd::cplx<T>* __restrict__ s,
-                                                               uint64_t n_tiles,
-                                                               const unsigned char* __restrict__ rec) {
+__device__ __forceinline__ void block_tile_body(ketd::cplx<T>* __restrict__ s, uint64_t n_tiles,
+                                                const unsigned char* __restrict__ rec,
+                                                uint64_t block, uint64_t nblocks) {
   constexpr int D = 1 << K;
   constexpr int NT = 1 << (TB - K);
   constexpr int SWB = sizeof(T) == 8 ? 3 : 4;
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(1 << (TB - K)) void k_block_tile(ketd::cplx<T>* __r
   const uint32_t tg = (uint32_t)deposit(threadIdx.x, mp.nloc);
   const uint32_t lg = swz(tg, mp);
   const uint64_t gg = deposit(tg, mp.pmask);
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (uint64_t tile = block; tile < n_tiles; tile += nblocks) {
     const uint64_t fixed = deposit_outside(tile, mp.pmask);
     io.load(s, fixed, L);
     __syncthreads();
@@ -552,10 +552,9 @@ __global__ __launch_bounds__(1 << (TB - K)) void k_block_tile(ketd::cplx<T>* __r
 // complex64 MFMA tiles: 256 threads, TB = 12; each wave applies the block
 // to TPW tiles of 16 groups with the operand layout of k_block_mfma.
 template <int K, bool PH>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 ? 3 : 1)))
-void k_block_tile_mfma(float2* __restrict__ s,
-                                                               uint64_t n_tiles,
-                                                               const unsigned char* __restrict__ rec) {
+__device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uint64_t n_tiles,
+                                                     const unsigned char* __restrict__ rec,
+                                                     uint64_t block, uint64_t nblocks) {
   constexpr int TB = 12;
   constexpr int D = 1 << K;
   constexpr int RB = 2 * D / 16;
@@ -615,7 +614,7 @@ void k_block_tile_mfma(float2* __restrict__ s,
   const int pg = threadIdx.x % G;
   const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / G);
   const uint64_t my_gg = PH ? deposit(deposit(pg, mp.nloc), mp.pmask) : 0;
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (uint64_t tile = block; tile < n_tiles; tile += nblocks) {
     const uint64_t fixed = deposit_outside(tile, mp.pmask);
     typename IO::V v[IO::S];
     io.fetch(s, fixed, v);
@@ -682,6 +681,37 @@ void k_block_tile_mfma(float2* __restrict__ s,
     io.store(s, fixed, L);
     __syncthreads();
   }
+}
+
+template <class T, int K, int TB, bool PH>
+__global__ __launch_bounds__(1 << (TB - K)) void k_block_tile(ketd::cplx<T>* __restrict__ s,
+                                                               uint64_t n_tiles,
+                                                               const unsigned char* __restrict__ rec) {
+  block_tile_body<T, K, TB, PH>(s, n_tiles, rec, blockIdx.x, gridDim.x);
+}
+
+// Batched: blockIdx.y = instance, as k_block_vec_batched.
+template <class T, int K, int TB, bool PH>
+__global__ __launch_bounds__(1 << (TB - K)) void k_block_tile_batched(
+    ketd::cplx<T>* __restrict__ s, uint64_t state_stride, uint64_t n_tiles,
+    const unsigned char* __restrict__ rec, uint64_t rec_stride) {
+  block_tile_body<T, K, TB, PH>(s + blockIdx.y * state_stride, n_tiles,
+                                rec + blockIdx.y * rec_stride, blockIdx.x, gridDim.x);
+}
+
+template <int K, bool PH>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 ? 3 : 1)))
+void k_block_tile_mfma(float2* __restrict__ s, uint64_t n_tiles,
+                       const unsigned char* __restrict__ rec) {
+  block_tile_mfma_body<K, PH>(s, n_tiles, rec, blockIdx.x, gridDim.x);
+}
+
+template <int K, bool PH>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 ? 3 : 1)))
+void k_block_tile_mfma_batched(float2* __restrict__ s, uint64_t state_stride, uint64_t n_tiles,
+                               const unsigned char* __restrict__ rec, uint64_t rec_stride) {
+  block_tile_mfma_body<K, PH>(s + blockIdx.y * state_stride, n_tiles,
+                              rec + blockIdx.y * rec_stride, blockIdx.x, gridDim.x);
 }
 
 inline unsigned capped_grid(uint64_t items, uint64_t cap) {
@@ -760,6 +790,38 @@ hipError_t block_tile_mfma(hipStream_t st, void* s, int n, int k, const unsigned
   return hipGetLastError();
 }
 
+// Batched tiles: up to 64 workgroups per instance along x (as the direct
+// batched kernel), grid-stride over the instance's tiles.
+template <class T, bool PH>
+hipError_t block_tile_batched(hipStream_t st, void* s, uint64_t state_stride, int n, int k,
+                              const unsigned char* rec, uint64_t rec_stride, unsigned instances) {
+  constexpr int TB = 11;
+  const uint64_t tiles = uint64_t{1} << (n - TB);
+  const dim3 grid((unsigned)(tiles < 64 ? tiles : 64), instances);
+  auto* p = static_cast<ketd::cplx<T>*>(s);
+  switch (k) {
+    case 3: k_block_tile_batched<T, 3, TB, PH><<<grid, 1 << (TB - 3), 0, st>>>(p, state_stride, tiles, rec, rec_stride); break;
+    case 4: k_block_tile_batched<T, 4, TB, PH><<<grid, 1 << (TB - 4), 0, st>>>(p, state_stride, tiles, rec, rec_stride); break;
+    case 5: k_block_tile_batched<T, 5, TB, PH><<<grid, 1 << (TB - 5), 0, st>>>(p, state_stride, tiles, rec, rec_stride); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <bool PH>
+hipError_t block_tile_mfma_batched(hipStream_t st, void* s, uint64_t state_stride, int n, int k,
+                                   const unsigned char* rec, uint64_t rec_stride,
+                                   unsigned instances) {
+  const uint64_t tiles = uint64_t{1} << (n - 12);
+  const dim3 grid((unsigned)(tiles < 64 ? tiles : 64), instances);
+  auto* p = static_cast<float2*>(s);
+  if (k == 4)
+    k_block_tile_mfma_batched<4, PH><<<grid, kThreads, 0, st>>>(p, state_stride, tiles, rec, rec_stride);
+  else
+    k_block_tile_mfma_batched<5, PH><<<grid, kThreads, 0, st>>>(p, state_stride, tiles, rec, rec_stride);
+  return hipGetLastError();
+}
+
 template <class T, bool PH>
 hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uint64_t groups,
                              int k, const unsigned char* rec, uint64_t rec_stride,
@@ -785,12 +847,26 @@ hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uin
 
 hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, int k,
                                 int n_terms, const void* rec_dev, uint64_t rec_stride,
-                                unsigned instances) {
+                                unsigned instances, int lowest) {
   if (k < 1 || k > 6 || k > n || n_terms < 0 || n_terms > kMaxTerms || instances == 0 ||
       instances > 65535)
     return hipErrorInvalidValue;
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
   const uint64_t dim = uint64_t{1} << n, groups = dim >> k;
+  const Tuning t = get_tuning();
+  const bool ph = n_terms > 0;
+  // The same choice as launch_block: MFMA tiles for complex64 k = 4, 5; LDS
+  // tiles for other widths 3..5 whose lowest qubit is below block_stage.
+  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= 12)
+    return ph ? block_tile_mfma_batched<true>(st, s, dim, n, k, rec, rec_stride, instances)
+              : block_tile_mfma_batched<false>(st, s, dim, n, k, rec, rec_stride, instances);
+  if (lowest >= 0 && lowest < t.block_stage && k >= 3 && k <= 5 && n >= 11) {
+    if (precision == 0)
+      return ph ? block_tile_batched<double, true>(st, s, dim, n, k, rec, rec_stride, instances)
+                : block_tile_batched<double, false>(st, s, dim, n, k, rec, rec_stride, instances);
+    return ph ? block_tile_batched<float, true>(st, s, dim, n, k, rec, rec_stride, instances)
+              : block_tile_batched<float, false>(st, s, dim, n, k, rec, rec_stride, instances);
+  }
   if (precision == 0)
     return n_terms ? block_vec_batched<double, true>(st, s, dim, groups, k, rec, rec_stride, instances)
                    : block_vec_batched<double, false>(st, s, dim, groups, k, rec, rec_stride, instances);

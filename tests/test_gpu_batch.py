@@ -69,3 +69,37 @@ def test_batch_rejects_bad_input():
         c = W.hea(3, 1)
         with pytest.raises(ValueError):
             b.run([c], [[(1.0, "XII")]])
+
+
+@pytest.fixture
+def restore_tuning():
+    t = N.get_tuning()
+    yield
+    N.set_tuning(**t)
+
+
+@pytest.mark.parametrize("precision", [N.C128, N.C64])
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
+    """HBM instances through the batched LDS-tile kernels (block_stage 64)
+    against the direct batched kernels (block_stage 0): the same bits where
+    both use the vector FMA path; complex64 k = 4, 5 tiles run on MFMA while
+    the direct batched kernel uses FP32 vector FMAs, so there both are held
+    to the oracle's tolerance only."""
+    ns = [13, 14, 15, 16, 17, 14, 15] if precision == N.C128 else [14, 15, 16, 17, 15, 16]
+    circs = [W.hea(n, 4, seed=900 + i) for i, n in enumerate(ns)]
+    obs = [zz_obs(n) for n in ns]
+    t = N.get_tuning()
+    out = {}
+    for stage in (0, 64):
+        N.set_tuning(**{**t, "block_stage": stage})
+        with N.Batch(ns, precision) as b:
+            vals, st = b.run(circs, obs, block_k=k, use_graph=False)
+            assert st["instances_hbm"] == len(ns)
+            out[stage] = vals
+    want = np.array([oracle_value(c, o) for c, o in zip(circs, obs)])
+    tol = 1e-10 if precision == N.C128 else 2e-4
+    if precision == N.C128 or k == 3:
+        assert np.array_equal(out[0], out[64])
+    assert np.abs(out[0] - want).max() <= tol
+    assert np.abs(out[64] - want).max() <= tol
