@@ -185,18 +185,23 @@ def bench_hea(args):
             for w in range(max(args.warmup, 1)):
                 b.run_packed(packed[w % 2], block_k=4, use_graph=True)
             t0 = time.perf_counter()
-            dev_ms, plan_s = 0.0, 0.0
+            dev_ms, plan_s, pack_s, up_s = 0.0, 0.0, 0.0, 0.0
             for k in range(args.steps):
                 vals, st = b.run_packed(packed[k % 2], block_k=4, use_graph=True)
                 dev_ms += st["device_ms"]
                 plan_s += st["plan_seconds"]
+                pack_s += st["pack_seconds"]
+                up_s += st["upload_seconds"]
             wall = (time.perf_counter() - t0) / args.steps
             res[label] = {"s_per_sweep": wall, "instances_per_s": len(ns) / wall,
                           "gates_per_s": gates / wall, "device_ms_per_sweep": dev_ms / args.steps,
-                          "plan_s_per_sweep": plan_s / args.steps, "launches": st["launches"],
+                          "plan_s_per_sweep": plan_s / args.steps,
+                          "pack_s_per_sweep": pack_s / args.steps,
+                          "upload_s_per_sweep": up_s / args.steps, "launches": st["launches"],
                           "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"]}
             log(f"[bench] {label}: {wall:.3f} s/sweep ({dev_ms / args.steps:.1f} ms device, "
-                f"{plan_s / args.steps:.3f} s planning), {len(ns) / wall:.0f} instances/s, "
+                f"{plan_s / args.steps:.3f} s planning of which {pack_s / args.steps:.3f} s "
+                f"packing, {up_s / args.steps:.3f} s upload), {len(ns) / wall:.0f} instances/s, "
                 f"{st['launches']} launches")
     cpu = None
     if not args.no_cpu:

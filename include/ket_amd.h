@@ -170,10 +170,13 @@ typedef struct ket_batch_stats {
   uint64_t launches;       /* kernel launches of the sweep */
   uint64_t gates;          /* unitary gates over all instances */
   double alg_bytes;        /* HBM bytes of the arena passes (line model) */
-  double plan_seconds;     /* host planning time */
+  double plan_seconds;     /* host time before the upload: plan + pack */
   double device_ms;        /* device time of the sweep (HIP events) */
   int32_t graph_replayed;  /* 1 when a captured hipGraph was replayed */
   int32_t pad;
+  double pack_seconds;     /* of plan_seconds: packing records and tables
+                              into the upload pool */
+  double upload_seconds;   /* pinned staging copy and transfer issue */
 } ket_batch_stats;
 
 /* n_instances states, instance i of n_qubits[i] (1..30) qubits. */

@@ -19,6 +19,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <string>
 #include <thread>
@@ -74,13 +75,67 @@ struct Launch {
 
 std::size_t align(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
 
-template <class T>
-std::size_t put(std::vector<unsigned char>& pool, const std::vector<T>& v) {
-  const std::size_t off = align(pool.size(), 256);
-  pool.resize(off + v.size() * sizeof(T));
-  if (!v.empty()) std::memcpy(pool.data() + off, v.data(), v.size() * sizeof(T));
-  return off;
+
+// fn(x) for x in [0, count) on up to 16 host threads.
+template <class F>
+void parallel_for(std::size_t count, F fn) {
+  unsigned T = std::thread::hardware_concurrency();
+  T = std::max(1u, std::min<unsigned>(T, 16));
+  T = (unsigned)std::min<std::size_t>(T, count);
+  std::atomic<std::size_t> next{0};
+  auto work = [&]() {
+    for (std::size_t x = next++; x < count; x = next++) fn(x);
+  };
+  std::vector<std::thread> pool_threads;
+  for (unsigned t = 1; t < T; ++t) pool_threads.emplace_back(work);
+  work();
+  for (std::thread& t : pool_threads) t.join();
 }
+
+// The upload pool of one sweep, laid out first and written once: small
+// tables are kept here until the upload; block records are copied straight
+// from the instances' plans into the pinned staging buffer, in parallel.
+class PoolBuilder {
+ public:
+  std::size_t size() const { return size_; }
+  std::size_t reserve(std::size_t bytes) {
+    const std::size_t off = align(size_, 256);
+    size_ = off + bytes;
+    return off;
+  }
+  template <class T>
+  std::size_t put(const std::vector<T>& v) {
+    const std::size_t bytes = v.size() * sizeof(T);
+    const std::size_t off = reserve(bytes);
+    if (bytes) {
+      const auto* p = reinterpret_cast<const unsigned char*>(v.data());
+      owned_.emplace_back(p, p + bytes);
+      jobs_.push_back({off, owned_.back().data(), bytes});
+    }
+    return off;
+  }
+  // `src` must stay valid until write_to().
+  void copy_later(std::size_t dst, const void* src, std::size_t bytes) {
+    jobs_.push_back({dst, src, bytes});
+  }
+  void write_to(unsigned char* dst) const {
+    const std::size_t per = 64, chunks = (jobs_.size() + per - 1) / per;
+    parallel_for(chunks, [&](std::size_t c) {
+      for (std::size_t j = c * per; j < std::min(jobs_.size(), (c + 1) * per); ++j)
+        std::memcpy(dst + jobs_[j].dst, jobs_[j].src, jobs_[j].n);
+    });
+  }
+
+ private:
+  struct Job {
+    std::size_t dst;
+    const void* src;
+    std::size_t n;
+  };
+  std::size_t size_ = 0;
+  std::vector<Job> jobs_;
+  std::deque<std::vector<unsigned char>> owned_;
+};
 
 ketk::LdsOp lds_op(const ketplan::Op& o) {
   ketk::LdsOp r{};
@@ -215,13 +270,41 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
   const auto t0 = std::chrono::steady_clock::now();
   const int P = b->precision;
   const std::size_t B = P == 0 ? 16 : 8;
-  std::vector<unsigned char> pool;
+  PoolBuilder pool;
   std::vector<Launch> launches;
   std::string key;
   std::vector<std::size_t> slot_of(ni);  // result slot -> instance
   std::size_t slot = 0;
   ket_batch_stats st{};
-  std::string err;
+
+  // ---- per-instance host work, all instances in parallel: the per-gate
+  // LDS ops of LDS-resident instances, the blocks-only plans of the others ----
+  std::vector<char> in_lds(ni, 0);
+  for (std::size_t i : b->lds_inst) in_lds[i] = 1;
+  std::vector<std::vector<ketk::LdsOp>> lds_ops(ni);
+  std::vector<ketplan::Plan> inst_plans(ni);
+  std::vector<int> status(ni, KET_OK);
+  std::vector<std::string> errs(ni);
+  parallel_for(ni, [&](std::size_t i) {
+    const int n = b->n[i];
+    const ket_gate* g = gates + gate_off[i];
+    const std::size_t ng = gate_off[i + 1] - gate_off[i];
+    if (!in_lds[i]) {
+      status[i] = plan_blocks_only(g, ng, n, P, block_k, &inst_plans[i], &errs[i]);
+      return;
+    }
+    for (std::size_t j = 0; j < ng; ++j) {
+      if (int s2 = ketplan::validate(g[j], n, &errs[i])) {
+        status[i] = s2;
+        return;
+      }
+      if (g[j].type < KET_H || g[j].type > KET_CSWAP) continue;
+      lds_ops[i].push_back(lds_op(ketplan::op_for_gate(g[j], n, P)));
+    }
+  });
+  for (std::size_t i = 0; i < ni; ++i)
+    if (status[i] != KET_OK) return fail(status[i], errs[i]);
+  const auto t_pack = std::chrono::steady_clock::now();
 
   // ---- LDS-resident instances, one launch per width ----
   for (std::size_t a = 0; a < b->lds_inst.size();) {
@@ -235,12 +318,8 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     for (std::size_t x = a; x < e; ++x) {
       const std::size_t i = b->lds_inst[x];
       nq.push_back(n);
-      for (uint64_t gi = gate_off[i]; gi < gate_off[i + 1]; ++gi) {
-        if (int s2 = ketplan::validate(gates[gi], n, &err)) return fail(s2, err);
-        if (gates[gi].type < KET_H || gates[gi].type > KET_CSWAP) continue;
-        ops.push_back(lds_op(ketplan::op_for_gate(gates[gi], n, P)));
-        ++st.gates;
-      }
+      ops.insert(ops.end(), lds_ops[i].begin(), lds_ops[i].end());
+      st.gates += lds_ops[i].size();
       opoff.push_back(ops.size());
       for (uint64_t t = term_off[i]; t < term_off[i + 1]; ++t) {
         zm.push_back(zmask[t]);
@@ -254,12 +333,12 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     L.instances = (unsigned)(e - a);
     L.max_qubits = n;
     L.out_off = slot;
-    L.nq_off = put(pool, nq);
-    L.ops_off = put(pool, ops);
-    L.opoff_off = put(pool, opoff);
-    L.z_off = put(pool, zm);
-    L.c_off = put(pool, cf);
-    L.toff_off = put(pool, toff);
+    L.nq_off = pool.put(nq);
+    L.ops_off = pool.put(ops);
+    L.opoff_off = pool.put(opoff);
+    L.z_off = pool.put(zm);
+    L.c_off = pool.put(cf);
+    L.toff_off = pool.put(toff);
     launches.push_back(L);
     key += "L" + std::to_string(n) + ":" + std::to_string(e - a) + ":" + std::to_string(ops.size()) +
            ":" + std::to_string(zm.size()) + ";";
@@ -270,32 +349,14 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
 
   // ---- HBM groups by width ----
   for (const auto& [n, inst] : b->hbm_groups) {
-    std::vector<ketplan::Plan> plans(inst.size());
-    std::vector<int> status(inst.size(), KET_OK);
-    std::vector<std::string> errs(inst.size());
-    {
-      // Instances plan independently: spread them over host threads.
-      unsigned T = std::thread::hardware_concurrency();
-      T = std::max(1u, std::min<unsigned>(T, 16));
-      T = (unsigned)std::min<std::size_t>(T, inst.size());
-      std::atomic<std::size_t> next{0};
-      auto work = [&]() {
-        for (std::size_t x = next++; x < inst.size(); x = next++) {
-          const std::size_t i = inst[x];
-          status[x] = plan_blocks_only(gates + gate_off[i], gate_off[i + 1] - gate_off[i], n, P,
-                                       block_k, &plans[x], &errs[x]);
-        }
-      };
-      std::vector<std::thread> pool_threads;
-      for (unsigned t = 1; t < T; ++t) pool_threads.emplace_back(work);
-      work();
-      for (std::thread& t : pool_threads) t.join();
-    }
+    // The plans stay in inst_plans: their records are copied at the upload.
+    auto plan_of = [&, &inst = inst](std::size_t x) -> const ketplan::Plan& {
+      return inst_plans[inst[x]];
+    };
     bool uniform = true;
     for (std::size_t x = 0; x < inst.size(); ++x) {
-      if (status[x] != KET_OK) return fail(status[x], errs[x]);
-      st.gates += plans[x].gates;
-      if (x && !same_structure(plans[0], plans[x])) uniform = false;
+      st.gates += plan_of(x).gates;
+      if (x && !same_structure(plan_of(0), plan_of(x))) uniform = false;
     }
     const std::size_t base = b->group_base.at(n);
     Launch F{};
@@ -306,18 +367,17 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     launches.push_back(F);
     key += "G" + std::to_string(n) + ":" + std::to_string(inst.size()) + ":" +
            (uniform ? "u" : "x") + ":";
-    const ketplan::Plan& p0 = plans[0];
+    const ketplan::Plan& p0 = plan_of(0);
     for (std::size_t op = 0; op < (uniform ? p0.ops.size() : 0); ++op) {
       // One record per instance, same stride.
       const std::size_t r0 = p0.ops[op].pool_offset;
       const auto* h0 = reinterpret_cast<const ketk::BlockRecord*>(p0.pool.data() + r0);
       const std::size_t D = std::size_t{1} << h0->k;
       const std::size_t rec = align(h0->mat_offset + D * D * B, 256);
-      const std::size_t off = align(pool.size(), 256);
-      pool.resize(off + rec * inst.size());
+      const std::size_t off = pool.reserve(rec * inst.size());
       for (std::size_t x = 0; x < inst.size(); ++x)
-        std::memcpy(pool.data() + off + x * rec, plans[x].pool.data() + plans[x].ops[op].pool_offset,
-                    h0->mat_offset + D * D * B);
+        pool.copy_later(off + x * rec, plan_of(x).pool.data() + plan_of(x).ops[op].pool_offset,
+                        h0->mat_offset + D * D * B);
       Launch L{};
       L.kind = Launch::Block;
       L.n = n;
@@ -334,13 +394,13 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     }
     if (!uniform) {
       for (std::size_t x = 0; x < inst.size(); ++x)
-        for (const ketplan::Op& o : plans[x].ops) {
-          const std::size_t off = align(pool.size(), 256);
-          const auto* h = reinterpret_cast<const ketk::BlockRecord*>(plans[x].pool.data() + o.pool_offset);
+        for (const ketplan::Op& o : plan_of(x).ops) {
+          const auto* h =
+              reinterpret_cast<const ketk::BlockRecord*>(plan_of(x).pool.data() + o.pool_offset);
           const std::size_t D = std::size_t{1} << h->k;
           const std::size_t bytes = h->mat_offset + D * D * B;
-          pool.resize(off + bytes);
-          std::memcpy(pool.data() + off, plans[x].pool.data() + o.pool_offset, bytes);
+          const std::size_t off = pool.reserve(bytes);
+          pool.copy_later(off, plan_of(x).pool.data() + o.pool_offset, bytes);
           Launch L{};
           L.kind = Launch::BlockSingle;
           L.n = n;
@@ -374,8 +434,8 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     E.n = n;
     E.instances = (unsigned)inst.size();
     E.arena_off = base;
-    E.z_off = put(pool, zm);
-    E.c_off = put(pool, cf);
+    E.z_off = pool.put(zm);
+    E.c_off = pool.put(cf);
     E.expval_terms = (int)nt;
     E.out_off = slot;
     launches.push_back(E);
@@ -385,10 +445,12 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     slot += inst.size();
   }
   key += "P" + std::to_string(pool.size());
-  st.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto t_up = std::chrono::steady_clock::now();
+  st.plan_seconds = std::chrono::duration<double>(t_up - t0).count();
+  st.pack_seconds = std::chrono::duration<double>(t_up - t_pack).count();
 
   // ---- upload the pool ----
-  if (pool.empty()) pool.resize(256);
+  if (pool.size() == 0) pool.reserve(256);
   if (b->pool_cap < pool.size()) {
     if (b->pool) (void)hipFree(b->pool);
     b->pool = nullptr;
@@ -406,9 +468,11 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     b->pool_host_cap = pool.size();
   }
   KET_HIP_TRY(hipStreamSynchronize(b->stream), "sync");
-  std::memcpy(b->pool_host, pool.data(), pool.size());
+  pool.write_to(static_cast<unsigned char*>(b->pool_host));
   KET_HIP_TRY(hipMemcpyAsync(b->pool, b->pool_host, pool.size(), hipMemcpyHostToDevice, b->stream),
               "hipMemcpyAsync(batch pool)");
+  st.upload_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_up).count();
   std::size_t need_work = 0;
   for (const Launch& L : launches)
     if (L.kind == Launch::Expval) need_work = std::max<std::size_t>(need_work, L.instances * 64);

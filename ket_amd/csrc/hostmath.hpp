@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <math.h>
+#include <array>
 #include <cstddef>
 #include <cstdint>
 #include <utility>
@@ -173,18 +174,147 @@ struct GateRef {
   int type;
   int local[3];
   const double* params;
+  const cx* mat2 = nullptr;  // if set: a one-qubit 2x2 (m00, m01, m10, m11) on local[0]
 };
+
+// Fused blocks (not ket's Fuser): each run of one-qubit gates on the same
+// block qubit, with no other gate on that qubit in between, becomes one 2x2
+// (later gates on the left), so block_matrix applies it to the 2^k columns
+// once. `store` holds the products and must outlive the returned refs.
+inline std::vector<GateRef> merge_single_runs(const std::vector<GateRef>& in,
+                                              std::vector<std::array<cx, 4>>& store) {
+  std::vector<GateRef> out;
+  std::vector<long> slot;          // out[i]'s product in `store`, or -1
+  long open[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // local qubit -> pending out index
+  store.clear();
+  for (const GateRef& g : in) {
+    if (arity(g.type) == 1) {
+      cx m[4];
+      single_matrix(g.type, g.params, m);
+      const int q = g.local[0];
+      if (open[q] >= 0) {
+        std::array<cx, 4>& a = store[(std::size_t)slot[(std::size_t)open[q]]];
+        a = {add(mul(m[0], a[0]), mul(m[1], a[2])), add(mul(m[0], a[1]), mul(m[1], a[3])),
+             add(mul(m[2], a[0]), mul(m[3], a[2])), add(mul(m[2], a[1]), mul(m[3], a[3]))};
+      } else {
+        store.push_back({m[0], m[1], m[2], m[3]});
+        open[q] = (long)out.size();
+        out.push_back(GateRef{KET_U, {q, 0, 0}, nullptr});
+        slot.push_back((long)store.size() - 1);
+      }
+      continue;
+    }
+    for (int j = 0; j < arity(g.type); ++j) open[g.local[j]] = -1;
+    out.push_back(g);
+    slot.push_back(-1);
+  }
+  for (std::size_t i = 0; i < out.size(); ++i)
+    if (slot[i] >= 0) out[i].mat2 = store[(std::size_t)slot[i]].data();
+  return out;
+}
+
+// All d columns advance together, each with apply_on_small's arithmetic (the
+// same products and sums per element), and each gate's 2x2 matrix or phase
+// is computed once instead of once per column.
 inline std::vector<cx> block_matrix(const std::vector<GateRef>& gates, int m) {
   const std::size_t d = std::size_t{1} << m;
-  std::vector<cx> mat(d * d);
-  std::vector<cx> e(d);
-  for (std::size_t col = 0; col < d; ++col) {
-    for (auto& x : e) x = {0.0, 0.0};
-    e[col] = {1.0, 0.0};
-    for (const GateRef& g : gates) apply_on_small(e, g.type, g.local, g.params);
-    for (std::size_t row = 0; row < d; ++row) mat[row * d + col] = e[row];
+  std::vector<cx> s(d * d, cx{0.0, 0.0});  // s[row * d + col]: column col
+  for (std::size_t col = 0; col < d; ++col) s[col * d + col] = {1.0, 0.0};
+  auto bit = [](int q) { return std::size_t{1} << q; };
+  auto single = [&](std::size_t tm, std::size_t cm, const cx mm[4]) {
+    for (std::size_t k = 0; k < d; ++k) {
+      if ((k & tm) || (k & cm) != cm) continue;
+      cx* a = &s[k * d];
+      cx* b = &s[(k | tm) * d];
+      for (std::size_t c = 0; c < d; ++c) {
+        const cx x = a[c], y = b[c];
+        a[c] = add(mul(mm[0], x), mul(mm[1], y));
+        b[c] = add(mul(mm[2], x), mul(mm[3], y));
+      }
+    }
+  };
+  auto swap_rows = [&](std::size_t k0, std::size_t k1) {
+    for (std::size_t c = 0; c < d; ++c) std::swap(s[k0 * d + c], s[k1 * d + c]);
+  };
+  for (const GateRef& g : gates) {
+    const int* on = g.local;
+    cx mm[4];
+    switch (g.type) {
+      case KET_H: case KET_X: case KET_Y: case KET_Z: case KET_S: case KET_SDG:
+      case KET_T: case KET_TDG: case KET_RX: case KET_RY: case KET_RZ: case KET_U:
+        if (g.mat2) {
+          single(bit(on[0]), 0, g.mat2);
+          break;
+        }
+        single_matrix(g.type, g.params, mm);
+        single(bit(on[0]), 0, mm);
+        break;
+      case KET_CRX: case KET_CRY: case KET_CRZ: case KET_CU:
+        single_matrix(g.type, g.params, mm);
+        single(bit(on[1]), bit(on[0]), mm);
+        break;
+      case KET_CH: {
+        const double r = 1.0 / std::sqrt(2.0);
+        const std::size_t cm = bit(on[0]), tm = bit(on[1]);
+        for (std::size_t k = 0; k < d; ++k) {
+          if (!(k & cm) || (k & tm)) continue;
+          cx* a = &s[k * d];
+          cx* b = &s[(k | tm) * d];
+          for (std::size_t c = 0; c < d; ++c) {
+            const cx x = a[c], y = b[c];
+            a[c] = {(x.re + y.re) * r, (x.im + y.im) * r};
+            b[c] = {(x.re - y.re) * r, (x.im - y.im) * r};
+          }
+        }
+        break;
+      }
+      case KET_CX: case KET_CCX: {
+        const int nc = g.type == KET_CX ? 1 : 2;
+        std::size_t cm = 0;
+        for (int i = 0; i < nc; ++i) cm |= bit(on[i]);
+        const std::size_t tm = bit(on[nc]);
+        for (std::size_t k = 0; k < d; ++k)
+          if ((k & cm) == cm && !(k & tm)) swap_rows(k, k | tm);
+        break;
+      }
+      case KET_CY: {
+        const std::size_t cm = bit(on[0]), tm = bit(on[1]);
+        for (std::size_t k = 0; k < d; ++k) {
+          if (!(k & cm) || (k & tm)) continue;
+          cx* a = &s[k * d];
+          cx* b = &s[(k | tm) * d];
+          for (std::size_t c = 0; c < d; ++c) {
+            const cx x = a[c], y = b[c];
+            a[c] = mul(cx{0.0, -1.0}, y);
+            b[c] = mul(cx{0.0, 1.0}, x);
+          }
+        }
+        break;
+      }
+      case KET_CZ: case KET_CP: {
+        const CosSin t = cos_sin(g.params[0]);
+        const cx ph = g.type == KET_CZ ? cx{-1.0, 0.0} : cx{t.c, t.s};
+        const std::size_t ma = bit(on[0]), mb = bit(on[1]);
+        for (std::size_t k = 0; k < d; ++k)
+          if ((k & ma) && (k & mb))
+            for (std::size_t c = 0; c < d; ++c) s[k * d + c] = mul(s[k * d + c], ph);
+        break;
+      }
+      case KET_SWAP: case KET_CSWAP: {
+        const int o = g.type == KET_SWAP ? 0 : 1;
+        const std::size_t cm = g.type == KET_SWAP ? 0 : bit(on[0]);
+        const std::size_t ma = bit(on[o]), mb = bit(on[o + 1]);
+        for (std::size_t k = 0; k < d; ++k) {
+          if ((k & cm) != cm) continue;
+          if ((k & ma) && !(k & mb)) swap_rows(k, (k ^ ma) | mb);
+        }
+        break;
+      }
+      default:
+        break;
+    }
   }
-  return mat;
+  return s;  // row-major: s[row * d + col]
 }
 
 }  // namespace kethost

@@ -474,12 +474,15 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
     std::vector<std::size_t> order(members);
     order.insert(order.end(), trailing.begin(), trailing.end());
     std::sort(order.begin(), order.end());
+    // The gates' own passes: the alternative a decider weighs (blocks-only
+    // planning without a decider never needs them).
     double sweeps = 0.0;
     std::vector<Op> singles;
-    for (std::size_t gi : order) {
-      singles.push_back(op_for_gate(gates[gi], n, precision));
-      sweeps += singles.back().line_fraction;
-    }
+    if (!force_blocks || decide)
+      for (std::size_t gi : order) {
+        singles.push_back(op_for_gate(gates[gi], n, precision));
+        sweeps += singles.back().line_fraction;
+      }
     const int k = __builtin_popcountll(Q);
     int low = 0;
     for (int qb = 0; qb < L; ++qb)
@@ -506,7 +509,8 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
           if (bq[b] == g.qubits[j]) r.local[j] = b;
       refs.push_back(r);
     }
-    const std::vector<cx> mat = kethost::block_matrix(refs, k);
+    std::vector<std::array<cx, 4>> merged;
+    const std::vector<cx> mat = kethost::block_matrix(kethost::merge_single_runs(refs, merged), k);
     const std::size_t pool_mark = plan->pool.size();
     Op block = (k == 1 && trailing.empty() && !force_blocks)
                    ? pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data())

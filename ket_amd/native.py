@@ -42,7 +42,8 @@ class BatchStats(ctypes.Structure):
                 ("launches", ctypes.c_uint64), ("gates", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_double), ("plan_seconds", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("graph_replayed", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("pack_seconds", ctypes.c_double),
+                ("upload_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}

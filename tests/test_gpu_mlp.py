@@ -69,3 +69,20 @@ def test_trained_model_gain_over_greedy():
         states.close()
     print("learned vs greedy:", res)
     assert res["speedup"] >= 0.9
+
+
+def test_learned_planner_beats_ket_fuser():
+    """The trained planner against ket's own fusion rule (KET_FUSE_REFERENCE,
+    the greedy Fuser of src/simulator.cpp:400-477) on held-out circuits of
+    the training distribution: whole-circuit wall time, both in complex64 on
+    resident states."""
+    mlp = CM.load()
+    states = CM.States(N.C64)
+    try:
+        circs = CM.config5_circuits(4, seed=4242, lo=26, hi=27, depth=40)
+        res = CM.evaluate(mlp, circs, states, ref_subset=len(circs))
+    finally:
+        states.close()
+    print("learned vs ket's Fuser:", res)
+    assert res["reference_fuser_circuits"] == len(circs)
+    assert res["speedup_vs_reference_fuser"] >= 2.0
