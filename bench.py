@@ -197,7 +197,9 @@ def bench_hea(args):
                           "gates_per_s": gates / wall, "device_ms_per_sweep": dev_ms / args.steps,
                           "plan_s_per_sweep": plan_s / args.steps,
                           "pack_s_per_sweep": pack_s / args.steps,
-                          "upload_s_per_sweep": up_s / args.steps, "launches": st["launches"],
+                          "upload_s_per_sweep": up_s / args.steps,
+                          "alg_GBps_device": st["alg_bytes"] / (st["device_ms"] / 1e3) / 1e9,
+                          "launches": st["launches"],
                           "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"]}
             log(f"[bench] {label}: {wall:.3f} s/sweep ({dev_ms / args.steps:.1f} ms device, "
                 f"{plan_s / args.steps:.3f} s planning of which {pack_s / args.steps:.3f} s "

@@ -181,3 +181,26 @@ def random_clifford(n: int, depth: int, seed: int, measures: int = 0) -> Circuit
         c.measure(int(rng.integers(n)), int(rng.integers(max(1, min(n, 64)))))
         k += 1
     return c
+
+
+_SELF_INVERSE = {"h", "x", "y", "z", "cx", "cy", "cz", "ch", "swap", "ccx", "cswap"}
+_SWAP_NAME = {"s": "sdg", "sdg": "s", "t": "tdg", "tdg": "t"}
+
+
+def inverse(c: Circuit) -> Circuit:
+    """The unitary part of `c` reversed and adjointed gate by gate:
+    U(t, p, l)^+ = U(-t, -l, -p), rotations and phases negate their angle."""
+    out = Circuit(c.n_qubits)
+    for name, qs, ps in reversed(c.ops):
+        if name in ("measure", "barrier"):
+            continue
+        if name in _SELF_INVERSE:
+            getattr(out, name)(*qs)
+        elif name in _SWAP_NAME:
+            getattr(out, _SWAP_NAME[name])(*qs)
+        elif name in ("u", "cu"):
+            t, p, l = ps
+            getattr(out, name)(*qs, -t, -l, -p)
+        else:  # rx, ry, rz, cp, crx, cry, crz
+            getattr(out, name)(*qs, *[-x for x in ps])
+    return out
