@@ -111,14 +111,13 @@ def dominant_kernel(records):
     return kind, d, avg_ms, avg_bytes, by
 
 
-def cpu_baseline(circ, threads, budget_s=20.0):
+def cpu_baseline(circ, threads, short=4, long_=24):
     """ket's own run() (reference library, oracle/_ref) on prefixes of the
     workload at its real size: rate = extra gates / extra time between a short
     and a longer prefix, so the 2^n allocation is not charged per gate."""
     from oracle import oracle as O
     if not O.ref_available():
         return None
-    short, long_ = 4, 24
     t_s = min(O.ref_time(circ.prefix(short), threads, 1)["seconds"])
     t_l = min(O.ref_time(circ.prefix(long_), threads, 1)["seconds"])
     rate = (long_ - short) / max(t_l - t_s, 1e-9)
@@ -166,6 +165,14 @@ def bench_brick33(args, circ, desc):
         "device": N.device_info()["name"]})
     plain.close()
     fused.close()
+    if not args.no_cpu:
+        # The reference on the real 2^33 shape (128 GiB of host memory): the
+        # first 2 and first 6 gates of the same circuit.
+        log(f"[bench] cpu baseline with {args.cpu_threads} threads (33-qubit prefixes)")
+        cpu = cpu_baseline(circ, args.cpu_threads, short=2, long_=6)
+        out["cpu_baseline"] = cpu
+        if cpu:
+            log(f"[bench] cpu: {cpu['value']:.3f} gates/s ({cpu['sample']})")
     return emit(out)
 
 
