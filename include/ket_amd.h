@@ -271,6 +271,10 @@ typedef struct ket_tuning {
                                whenever this is > 0, other blocks of width
                                3..5 whose lowest qubit is below it (0: never;
                                64: always; default 3) */
+  int32_t pair_exchange;    /* pair passes whose partner differs in index
+                               bits below this run one amplitude per thread,
+                               the two lanes of a pair swapping theirs (0:
+                               never; 64: always; default 16) */
 } ket_tuning;
 int ket_set_tuning(const ket_tuning* t);
 int ket_get_tuning(ket_tuning* t);

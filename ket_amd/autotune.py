@@ -25,7 +25,8 @@ from .gates import Circuit
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-SPACE = {"items_per_thread": [1, 2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32]}
+SPACE = {"items_per_thread": [1, 2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32],
+         "pair_exchange": [0, 16]}
 
 
 def probe_circuit(n: int) -> Circuit:
@@ -113,16 +114,16 @@ def tune(n: int = 30, precision: int = N.C128, reps: int = 3, log=print) -> dict
     circ = probe_circuit(n)
     results = []
     with N.StateVector(n, precision) as sv:
-        for ipt, nt, bpc in itertools.product(*SPACE.values()):
-            N.set_tuning(items_per_thread=ipt, nontemporal=nt, blocks_per_cu=bpc)
+        for point in itertools.product(*SPACE.values()):
+            cfg = dict(zip(SPACE, point))
+            N.set_tuning(**cfg)
             m = measure(sv, circ, reps)
-            m.update({"items_per_thread": ipt, "nontemporal": nt, "blocks_per_cu": bpc})
+            m.update(cfg)
             results.append(m)
-            log(f"[tune] ipt={ipt} nt={nt} bpc={bpc}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
+            log(f"[tune] {cfg}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 + " ".join(f"{k}={v:.0f}" for k, v in m["by_kind_GBps"].items()))
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(items_per_thread=best["items_per_thread"], nontemporal=best["nontemporal"],
-                 blocks_per_cu=best["blocks_per_cu"])
+    N.set_tuning(**{k: best[k] for k in SPACE})
     return {"device": N.device_info()["name"], "precision": precision, "qubits": n,
             "best": {k: best[k] for k in SPACE}, "best_GBps": best["GBps"], "all": results}
 
@@ -136,8 +137,7 @@ def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
     entry = table.get("c128" if precision == N.C128 else "c64")
     if entry:
         b = entry["best"]
-        N.set_tuning(items_per_thread=b["items_per_thread"], nontemporal=b["nontemporal"],
-                     blocks_per_cu=b["blocks_per_cu"])
+        N.set_tuning(**{k: b[k] for k in SPACE if k in b})
         return b
     return None
 

@@ -701,9 +701,11 @@ int ket_set_tuning(const ket_tuning* t) {
     return fail(KET_ERR_ARG, "ket_amd: block_tiles must be 0..2, block_bpc 0..64");
   if (t->block_stage < 0 || t->block_stage > 64)
     return fail(KET_ERR_ARG, "ket_amd: block_stage must be in [0, 64]");
+  if (t->pair_exchange < 0 || t->pair_exchange > 64)
+    return fail(KET_ERR_ARG, "ket_amd: pair_exchange must be in [0, 64]");
   ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu,
                     t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
-                    t->block_stage});
+                    t->block_stage, t->pair_exchange});
   return KET_OK;
 }
 
@@ -716,6 +718,7 @@ int ket_get_tuning(ket_tuning* t) {
   t->block_tiles = k.block_tiles;
   t->block_bpc = k.block_bpc;
   t->block_stage = k.block_stage;
+  t->pair_exchange = k.pair_exchange;
   return KET_OK;
 }
 

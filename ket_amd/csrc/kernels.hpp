@@ -32,6 +32,8 @@ struct Tuning {
   int block_tiles;   // MFMA block kernel: 16-group tiles per wave iteration (1, 2)
   int block_bpc;     // block kernels: workgroups per CU (grid-stride beyond)
   int block_stage;   // blocks whose lowest qubit is below this go through LDS tiles
+  int pair_exchange; // pair passes whose partner's lowest index bit is below
+                     // this: one amplitude per thread, lane-pair swap
 };
 void set_tuning(const Tuning& t);
 Tuning get_tuning();

@@ -110,6 +110,58 @@ __global__ __launch_bounds__(kThreads) void k_pair(cplx<T>* __restrict__ s,
   }
 }
 
+// Value of the neighbouring lane (lane ^ 1): DPP quad_perm [1,0,3,2], a
+// register move across adjacent lanes with no LDS traffic.
+__device__ __forceinline__ int lane_swap(int x) {
+  return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_swap(float x) {
+  return __int_as_float(lane_swap(__float_as_int(x)));
+}
+__device__ __forceinline__ double lane_swap(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = lane_swap((int)(v & 0xffffffffLL));
+  const int hi = lane_swap((int)(v >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Pair passes with one amplitude per thread: lanes 2m and 2m+1 hold the two
+// amplitudes of a pair (side 0 the index `lo`, side 1 its partner), swap
+// them across the lane pair, and each computes and stores only its own
+// output with the same products as k_pair. One load and one store per
+// thread, the access shape of the fastest copy.
+template <class T, int OP, int NT>
+__global__ __launch_bounds__(kThreads) void k_pair_x(cplx<T>* __restrict__ s, PairSpec sp,
+                                                      cplx<T> m00, cplx<T> m01, cplx<T> m10,
+                                                      cplx<T> m11) {
+  const uint64_t n2 = sp.n_items * 2;
+  // Both lanes of a pair share j >> 1, so they enter and leave the loop
+  // together and the cross-lane swap always has its partner lane active.
+  for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < n2;
+       j += (uint64_t)gridDim.x * kThreads) {
+    const bool side = j & 1;
+    const uint64_t lo = insert_zeros(j >> 1, sp.npos, sp.pos) | sp.set_mask;
+    const uint64_t e = side ? (lo ^ sp.partner) : lo;
+    const cplx<T> v = ld<NT>(s + e);
+    cplx<T> w;
+    w.x = lane_swap(v.x);
+    w.y = lane_swap(v.y);
+    const cplx<T> a = side ? w : v, b = side ? v : w;
+    cplx<T> o;
+    if constexpr (OP == OP_DENSE) {
+      o = side ? cadd<T>(cmul<T>(m10, a), cmul<T>(m11, b))
+               : cadd<T>(cmul<T>(m00, a), cmul<T>(m01, b));
+    } else if constexpr (OP == OP_CH) {
+      const T r = m00.x;
+      o = side ? ketd::mk<T>((a.x - b.x) * r, (a.y - b.y) * r)
+               : ketd::mk<T>((a.x + b.x) * r, (a.y + b.y) * r);
+    } else {  // OP_SWAP
+      o = w;
+    }
+    st<NT>(s + e, o);
+  }
+}
+
 template <class T, int ITEMS, int NT>
 __global__ __launch_bounds__(kThreads) void k_phase(cplx<T>* __restrict__ s,
                                                      PairSpec sp, cplx<T> ph) {
@@ -201,9 +253,11 @@ cplx<T> to_c(const double* m, int i) {
 }
 
 // Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json):
-// one pair per thread over a full grid; fused blocks with 32 workgroups per CU,
-// LDS tiles for complex64 MFMA blocks and for blocks on qubits 0..2.
-Tuning g_tuning{1, 0, 0, 1, 32, 3};
+// one pair per thread over a full grid, or one amplitude per thread with a
+// lane-pair swap when the partner is fewer than 2^16 amplitudes away; fused
+// blocks with 32 workgroups per CU, LDS tiles for complex64 MFMA blocks and
+// for blocks on qubits 0..2.
+Tuning g_tuning{1, 0, 0, 1, 32, 3, 16};
 
 inline unsigned tuned_grid(uint64_t items, int per_thread) {
   const uint64_t per_block = (uint64_t)kThreads * per_thread;
@@ -224,9 +278,22 @@ void pair_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) 
                                                       to_c<T>(m, 2), to_c<T>(m, 3));
 }
 
+template <class T, int OP, int NT>
+void pair_x_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
+  const unsigned grid = tuned_grid(sp.n_items * 2, 1);
+  k_pair_x<T, OP, NT><<<grid, kThreads, 0, st>>>(p, sp, to_c<T>(m, 0), to_c<T>(m, 1),
+                                                 to_c<T>(m, 2), to_c<T>(m, 3));
+}
+
 template <class T, int OP>
 void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
   const bool nt = g_tuning.nontemporal != 0;
+  // The exchange form for near partners: with a far partner (2^16 amplitudes
+  // or more) interleaving both halves in one instruction measured slower.
+  if (sp.partner && __builtin_ctzll(sp.partner) < g_tuning.pair_exchange) {
+    nt ? pair_x_go<T, OP, 1>(st, p, sp, m) : pair_x_go<T, OP, 0>(st, p, sp, m);
+    return;
+  }
   switch (g_tuning.items_per_thread) {
     case 1: nt ? pair_go<T, OP, 1, 1>(st, p, sp, m) : pair_go<T, OP, 1, 0>(st, p, sp, m); break;
     case 2: nt ? pair_go<T, OP, 2, 1>(st, p, sp, m) : pair_go<T, OP, 2, 0>(st, p, sp, m); break;

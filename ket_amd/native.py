@@ -142,11 +142,12 @@ def device_info() -> dict:
 class Tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("block_tiles", ctypes.c_int32),
-                ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32)]
+                ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32),
+                ("pair_exchange", ctypes.c_int32)]
 
 
 TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tiles", "block_bpc",
-                 "block_stage")
+                 "block_stage", "pair_exchange")
 
 
 def set_tuning(**changes):

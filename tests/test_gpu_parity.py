@@ -51,6 +51,36 @@ def test_each_gate_every_placement_exact(name):
         assert np.array_equal(got, want), (name, qs, np.abs(got - want).max())
 
 
+@pytest.mark.parametrize("name", GATES)
+def test_pair_exchange_form_exact(name):
+    """Pair passes with one amplitude per thread (lane-pair swap) give ket's
+    values in complex128 and the same bits as the two-amplitude form in
+    complex64, at every tested placement."""
+    n = 7
+    k = ARITY[name]
+    rng = np.random.default_rng(hash(name) % 1000 + 7)
+    placements = [p for p in itertools.permutations(range(n), k)]
+    rng.shuffle(placements)
+    placements = placements[:12] + [tuple(range(k)), tuple(range(n - k, n))]
+    s0 = random_state(n, 5)
+    t = N.get_tuning()
+    try:
+        for qs in placements:
+            c = Circuit(n)
+            params = list(rng.uniform(0, 2 * np.pi, NPARAMS.get(name, 0)))
+            getattr(c, name)(*qs, *params)
+            want = O.run(c, fuse=False, state=s0)
+            N.set_tuning(**{**t, "pair_exchange": 64})
+            got = gpu_run(c, N.FUSE_NONE, state=s0)
+            got64 = gpu_run(c, N.FUSE_NONE, N.C64, state=s0.astype(np.complex64))
+            N.set_tuning(**{**t, "pair_exchange": 0})
+            ref64 = gpu_run(c, N.FUSE_NONE, N.C64, state=s0.astype(np.complex64))
+            assert np.array_equal(got, want), (name, qs, np.abs(got - want).max())
+            assert np.array_equal(got64, ref64), (name, qs)
+    finally:
+        N.set_tuning(**t)
+
+
 CIRCUITS = {
     "mixed12": lambda: W.random_mixed(12, 14, 7),
     "brick14": lambda: W.brickwork(14, 10, seed=2),
