@@ -259,7 +259,9 @@ int ket_mlp_use_for_planning(ket_mlp* m);
  *      no launch shape) ---- */
 typedef struct ket_tuning {
   int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 1, 2, 4, 8 */
-  int32_t nontemporal;      /* 1: streaming (nontemporal) loads and stores */
+  int32_t nontemporal;      /* gate passes' loads and stores: 0 plain, 1
+                               nontemporal, 2 nontemporal when every 128-byte
+                               line the pass writes is written whole */
   int32_t blocks_per_cu;    /* 0: one block per 256*items items; else cap the
                                grid at 256*blocks_per_cu blocks (grid-stride) */
   int32_t block_tiles;      /* MFMA blocks: 16-group tiles each wave loads
@@ -274,10 +276,14 @@ typedef struct ket_tuning {
   int32_t pair_exchange;    /* pair passes whose partner differs in index
                                bits below this run one amplitude per thread,
                                the two lanes of a pair swapping theirs (0:
-                               never; 64: always; default 16) */
+                               never; 64: always) */
 } ket_tuning;
+/* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
+ * The _for forms address one precision (KET_C128 or KET_C64). */
 int ket_set_tuning(const ket_tuning* t);
 int ket_get_tuning(ket_tuning* t);
+int ket_set_tuning_for(int precision, const ket_tuning* t);
+int ket_get_tuning_for(int precision, ket_tuning* t);
 /* Achievable HBM bandwidth on this device: a plain copy kernel (the
  * per-gate passes' launch shape) of `bytes` between two device buffers,
  * `reps` times; *gbps = 2 * bytes * reps / time / 1e9 (read plus write).

@@ -25,8 +25,8 @@ from .gates import Circuit
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-SPACE = {"items_per_thread": [1, 2, 4, 8], "nontemporal": [0, 1], "blocks_per_cu": [0, 8, 32],
-         "pair_exchange": [0, 16]}
+SPACE = {"items_per_thread": [1, 2, 4], "nontemporal": [0, 1, 2], "blocks_per_cu": [0, 32],
+         "pair_exchange": [0, 4, 16]}
 
 
 def probe_circuit(n: int) -> Circuit:
@@ -73,7 +73,7 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
     with N.StateVector(n, precision) as sv:
         for point in itertools.product(*BLOCK_SPACE.values()):
             cfg = dict(zip(BLOCK_SPACE, point))
-            N.set_tuning(**cfg)
+            N.set_tuning(precision, **cfg)
             sv.profile_circuit(circ, N.FUSE_BLOCKS_ALL, k)
             ms = []
             for _ in range(reps):
@@ -90,7 +90,7 @@ def tune_blocks(n: int = 30, precision: int = N.C64, reps: int = 3, log=print) -
             log(f"[tune] blocks k={k} {cfg}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 f"by lowest qubit {m['per_block_GBps']}")
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(**{kk: best[kk] for kk in BLOCK_SPACE})
+    N.set_tuning(precision, **{kk: best[kk] for kk in BLOCK_SPACE})
     return {"k": k, "best": {kk: best[kk] for kk in BLOCK_SPACE}, "best_GBps": best["GBps"],
             "all": results}
 
@@ -116,14 +116,14 @@ def tune(n: int = 30, precision: int = N.C128, reps: int = 3, log=print) -> dict
     with N.StateVector(n, precision) as sv:
         for point in itertools.product(*SPACE.values()):
             cfg = dict(zip(SPACE, point))
-            N.set_tuning(**cfg)
+            N.set_tuning(precision, **cfg)
             m = measure(sv, circ, reps)
             m.update(cfg)
             results.append(m)
             log(f"[tune] {cfg}: {m['ms']:.2f} ms, {m['GBps']:.0f} GB/s "
                 + " ".join(f"{k}={v:.0f}" for k, v in m["by_kind_GBps"].items()))
     best = min(results, key=lambda r: r["ms"])
-    N.set_tuning(**{k: best[k] for k in SPACE})
+    N.set_tuning(precision, **{k: best[k] for k in SPACE})
     return {"device": N.device_info()["name"], "precision": precision, "qubits": n,
             "best": {k: best[k] for k in SPACE}, "best_GBps": best["GBps"], "all": results}
 
@@ -137,7 +137,7 @@ def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
     entry = table.get("c128" if precision == N.C128 else "c64")
     if entry:
         b = entry["best"]
-        N.set_tuning(**{k: b[k] for k in SPACE if k in b})
+        N.set_tuning(precision, **{k: b[k] for k in SPACE if k in b})
         return b
     return None
 

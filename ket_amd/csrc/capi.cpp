@@ -690,11 +690,14 @@ int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity, double* max_abs_diff
   return KET_OK;
 }
 
-int ket_set_tuning(const ket_tuning* t) {
+namespace {
+int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
   if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
   if (t->items_per_thread != 1 && t->items_per_thread != 2 && t->items_per_thread != 4 &&
       t->items_per_thread != 8)
     return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 1, 2, 4 or 8");
+  if (t->nontemporal < 0 || t->nontemporal > 2)
+    return fail(KET_ERR_ARG, "ket_amd: nontemporal must be 0, 1 or 2");
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
     return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
   if (t->block_tiles < 0 || t->block_tiles > 2 || t->block_bpc < 0 || t->block_bpc > 64)
@@ -703,15 +706,13 @@ int ket_set_tuning(const ket_tuning* t) {
     return fail(KET_ERR_ARG, "ket_amd: block_stage must be in [0, 64]");
   if (t->pair_exchange < 0 || t->pair_exchange > 64)
     return fail(KET_ERR_ARG, "ket_amd: pair_exchange must be in [0, 64]");
-  ketk::set_tuning({t->items_per_thread, t->nontemporal ? 1 : 0, t->blocks_per_cu,
-                    t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
-                    t->block_stage, t->pair_exchange});
+  *out = {t->items_per_thread, t->nontemporal, t->blocks_per_cu,
+          t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
+          t->block_stage, t->pair_exchange};
   return KET_OK;
 }
 
-int ket_get_tuning(ket_tuning* t) {
-  if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
-  const ketk::Tuning k = ketk::get_tuning();
+void fill_tuning(const ketk::Tuning& k, ket_tuning* t) {
   t->items_per_thread = k.items_per_thread;
   t->nontemporal = k.nontemporal;
   t->blocks_per_cu = k.blocks_per_cu;
@@ -719,6 +720,37 @@ int ket_get_tuning(ket_tuning* t) {
   t->block_bpc = k.block_bpc;
   t->block_stage = k.block_stage;
   t->pair_exchange = k.pair_exchange;
+}
+}  // namespace
+
+int ket_set_tuning(const ket_tuning* t) {
+  ketk::Tuning k;
+  if (int st = check_tuning(t, &k)) return st;
+  ketk::set_tuning(0, k);
+  ketk::set_tuning(1, k);
+  return KET_OK;
+}
+
+int ket_get_tuning(ket_tuning* t) {
+  if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
+  fill_tuning(ketk::get_tuning(0), t);
+  return KET_OK;
+}
+
+int ket_set_tuning_for(int precision, const ket_tuning* t) {
+  if (precision != KET_C128 && precision != KET_C64)
+    return fail(KET_ERR_ARG, "ket_amd: unknown precision");
+  ketk::Tuning k;
+  if (int st = check_tuning(t, &k)) return st;
+  ketk::set_tuning(precision, k);
+  return KET_OK;
+}
+
+int ket_get_tuning_for(int precision, ket_tuning* t) {
+  if (precision != KET_C128 && precision != KET_C64)
+    return fail(KET_ERR_ARG, "ket_amd: unknown precision");
+  if (!t) return fail(KET_ERR_ARG, "ket_amd: null tuning");
+  fill_tuning(ketk::get_tuning(precision), t);
   return KET_OK;
 }
 

@@ -35,8 +35,9 @@ struct Tuning {
   int pair_exchange; // pair passes whose partner's lowest index bit is below
                      // this: one amplitude per thread, lane-pair swap
 };
-void set_tuning(const Tuning& t);
-Tuning get_tuning();
+// Per precision (0 = complex128, 1 = complex64).
+void set_tuning(int precision, const Tuning& t);
+Tuning get_tuning(int precision);
 hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes,
                        int nt, int blocks);
 

@@ -723,7 +723,7 @@ inline unsigned capped_grid(uint64_t items, uint64_t cap) {
 
 template <class T, bool PH>
 hipError_t block_vec(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(sizeof(T) == 8 ? 0 : 1);
   const unsigned g = capped_grid(groups, 256 * (uint64_t)(t.block_bpc > 0 ? 2 * t.block_bpc : 64));
   auto* p = static_cast<ketd::cplx<T>*>(s);
   switch (k) {
@@ -742,7 +742,7 @@ template <bool PH>
 hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const unsigned char* rec) {
   // Waves stream tiles of 16 groups; a capped grid (block_bpc workgroups
   // per CU) keeps the matrix registers loaded once per wave.
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(1);
   const int tpi = t.block_tiles == 2 ? 2 : 1;
   const uint64_t tiles = groups / 16;
   uint64_t blocks = (tiles + 4 * tpi - 1) / (4 * tpi);
@@ -764,7 +764,7 @@ hipError_t block_mfma(hipStream_t st, void* s, uint64_t groups, int k, const uns
 template <class T, bool PH>
 hipError_t block_tile(hipStream_t st, void* s, int n, int k, const unsigned char* rec) {
   constexpr int TB = 11;
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(sizeof(T) == 8 ? 0 : 1);
   const uint64_t tiles = uint64_t{1} << (n - TB);
   const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
   const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
@@ -780,7 +780,7 @@ hipError_t block_tile(hipStream_t st, void* s, int n, int k, const unsigned char
 
 template <bool PH>
 hipError_t block_tile_mfma(hipStream_t st, void* s, int n, int k, const unsigned char* rec) {
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(1);
   const uint64_t tiles = uint64_t{1} << (n - 12);
   const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
   const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
@@ -853,7 +853,7 @@ hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, i
     return hipErrorInvalidValue;
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
   const uint64_t dim = uint64_t{1} << n, groups = dim >> k;
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(precision);
   const bool ph = n_terms > 0;
   // The same choice as launch_block: MFMA tiles for complex64 k = 4, 5; LDS
   // tiles for other widths 3..5 whose lowest qubit is below block_stage.
@@ -880,7 +880,7 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
   const uint64_t groups = (uint64_t{1} << n) >> k;
   const bool ph = n_terms > 0;
-  const Tuning t = get_tuning();
+  const Tuning t = get_tuning(precision);
   // MFMA blocks: LDS tiles at any position (block_stage > 0); vector
   // blocks: only when their lowest qubit is below block_stage, since the
   // direct kernel's group loads coalesce once the low qubits are outside.

@@ -252,18 +252,32 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
-// Default: the autotuner's pick on MI355X for both precisions (ket_amd/tuning.json):
-// one pair per thread over a full grid, or one amplitude per thread with a
-// lane-pair swap when the partner is fewer than 2^16 amplitudes away; fused
-// blocks with 32 workgroups per CU, LDS tiles for complex64 MFMA blocks and
-// for blocks on qubits 0..2.
-Tuning g_tuning{1, 0, 0, 1, 32, 3, 16};
+// Defaults, per precision (complex128, complex64): the autotuner's picks on
+// MI355X (ket_amd/tuning.json). Gate passes: one pair per thread over a full
+// grid; nontemporal access when every 128-byte line a pass writes is written
+// whole (mode 2); the lane-pair exchange form for partners fewer than 2^4
+// amplitudes away. Fused blocks: 32 workgroups per CU, LDS tiles for
+// complex64 MFMA blocks and for blocks on qubits 0..2.
+Tuning g_tuning[2] = {{1, 2, 0, 1, 32, 3, 4}, {1, 2, 0, 1, 32, 3, 4}};
 
-inline unsigned tuned_grid(uint64_t items, int per_thread) {
+// True when every 128-byte line the pass writes is written whole: no
+// condition bit (control, phase bit, the bits of a SWAP) below the line's
+// index bits. Nontemporal stores of partial lines measured 1.5-2x slower.
+inline bool whole_lines(const PairSpec& sp, bool swap2, int line_bits) {
+  uint64_t cond = sp.set_mask;
+  if (swap2) cond |= sp.partner;
+  return (cond & ((uint64_t{1} << line_bits) - 1)) == 0;
+}
+
+inline bool use_nt(const Tuning& t, const PairSpec& sp, bool swap2, int line_bits) {
+  return t.nontemporal == 1 || (t.nontemporal == 2 && whole_lines(sp, swap2, line_bits));
+}
+
+inline unsigned tuned_grid(const Tuning& t, uint64_t items, int per_thread) {
   const uint64_t per_block = (uint64_t)kThreads * per_thread;
   uint64_t g = (items + per_block - 1) / per_block;
-  if (g_tuning.blocks_per_cu > 0) {
-    const uint64_t cap = (uint64_t)256 * g_tuning.blocks_per_cu;
+  if (t.blocks_per_cu > 0) {
+    const uint64_t cap = (uint64_t)256 * t.blocks_per_cu;
     if (g > cap) g = cap;
   }
   if (g > (uint64_t{1} << 22)) g = uint64_t{1} << 22;  // 2^30 work-items
@@ -271,30 +285,37 @@ inline unsigned tuned_grid(uint64_t items, int per_thread) {
   return (unsigned)g;
 }
 
+template <class T>
+const Tuning& tuning_of() {
+  return g_tuning[sizeof(T) == 8 ? 0 : 1];
+}
+
 template <class T, int OP, int ITEMS, int NT>
 void pair_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
-  const unsigned grid = tuned_grid(sp.n_items, ITEMS);
+  const unsigned grid = tuned_grid(tuning_of<T>(), sp.n_items, ITEMS);
   k_pair<T, OP, ITEMS, NT><<<grid, kThreads, 0, st>>>(p, sp, to_c<T>(m, 0), to_c<T>(m, 1),
                                                       to_c<T>(m, 2), to_c<T>(m, 3));
 }
 
 template <class T, int OP, int NT>
 void pair_x_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
-  const unsigned grid = tuned_grid(sp.n_items * 2, 1);
+  const unsigned grid = tuned_grid(tuning_of<T>(), sp.n_items * 2, 1);
   k_pair_x<T, OP, NT><<<grid, kThreads, 0, st>>>(p, sp, to_c<T>(m, 0), to_c<T>(m, 1),
                                                  to_c<T>(m, 2), to_c<T>(m, 3));
 }
 
 template <class T, int OP>
 void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
-  const bool nt = g_tuning.nontemporal != 0;
-  // The exchange form for near partners: with a far partner (2^16 amplitudes
-  // or more) interleaving both halves in one instruction measured slower.
-  if (sp.partner && __builtin_ctzll(sp.partner) < g_tuning.pair_exchange) {
+  const Tuning& t = tuning_of<T>();
+  const bool nt = use_nt(t, sp, OP == OP_SWAP && __builtin_popcountll(sp.partner) > 1,
+                         sizeof(T) == 8 ? 3 : 4);
+  // The exchange form for near partners; with far partners interleaving both
+  // halves in one instruction measured slower.
+  if (sp.partner && __builtin_ctzll(sp.partner) < t.pair_exchange) {
     nt ? pair_x_go<T, OP, 1>(st, p, sp, m) : pair_x_go<T, OP, 0>(st, p, sp, m);
     return;
   }
-  switch (g_tuning.items_per_thread) {
+  switch (t.items_per_thread) {
     case 1: nt ? pair_go<T, OP, 1, 1>(st, p, sp, m) : pair_go<T, OP, 1, 0>(st, p, sp, m); break;
     case 2: nt ? pair_go<T, OP, 2, 1>(st, p, sp, m) : pair_go<T, OP, 2, 0>(st, p, sp, m); break;
     case 8: nt ? pair_go<T, OP, 8, 1>(st, p, sp, m) : pair_go<T, OP, 8, 0>(st, p, sp, m); break;
@@ -316,14 +337,16 @@ hipError_t pair_impl(hipStream_t st, void* s, const PairSpec& sp, int op,
 
 template <class T, int ITEMS, int NT>
 void phase_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, cplx<T> ph) {
-  k_phase<T, ITEMS, NT><<<tuned_grid(sp.n_items, ITEMS), kThreads, 0, st>>>(p, sp, ph);
+  k_phase<T, ITEMS, NT><<<tuned_grid(tuning_of<T>(), sp.n_items, ITEMS), kThreads, 0, st>>>(
+      p, sp, ph);
 }
 
 template <class T>
 hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
   auto* p = static_cast<cplx<T>*>(s);
-  const bool nt = g_tuning.nontemporal != 0;
-  switch (g_tuning.items_per_thread) {
+  const Tuning& t = tuning_of<T>();
+  const bool nt = use_nt(t, sp, false, sizeof(T) == 8 ? 3 : 4);
+  switch (t.items_per_thread) {
     case 1: nt ? phase_go<T, 1, 1>(st, p, sp, ph) : phase_go<T, 1, 0>(st, p, sp, ph); break;
     case 2: nt ? phase_go<T, 2, 1>(st, p, sp, ph) : phase_go<T, 2, 0>(st, p, sp, ph); break;
     case 8: nt ? phase_go<T, 8, 1>(st, p, sp, ph) : phase_go<T, 8, 0>(st, p, sp, ph); break;
@@ -369,8 +392,8 @@ hipError_t launch_phase(hipStream_t st, int precision, void* s,
              : phase_impl<float>(st, s, sp, ketd::mk<float>((float)re, (float)im));
 }
 
-void set_tuning(const Tuning& t) { g_tuning = t; }
-Tuning get_tuning() { return g_tuning; }
+void set_tuning(int precision, const Tuning& t) { g_tuning[precision == 0 ? 0 : 1] = t; }
+Tuning get_tuning(int precision) { return g_tuning[precision == 0 ? 0 : 1]; }
 
 hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes, int nt,
                        int blocks) {

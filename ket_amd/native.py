@@ -6,6 +6,7 @@ the HIP extension fails loudly instead of falling back to anything else.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 
@@ -98,6 +99,8 @@ def lib():
             "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
             "ket_sv_profile_circuit": ([vp, vp, sz, i32, i32, vp, sz, P(sz)], i32),
             "ket_set_tuning": ([vp], i32),
+            "ket_set_tuning_for": ([i32, vp], i32),
+            "ket_get_tuning_for": ([i32, vp], i32),
             "ket_get_tuning": ([vp], i32),
             "ket_calibrate_copy": ([u64, i32, i32, P(dbl)], i32),
             "ket_batch_create": ([sz, vp, i32, P(vp)], i32),
@@ -150,21 +153,36 @@ TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tile
                  "block_stage", "pair_exchange")
 
 
-def set_tuning(**changes):
-    """Sets the named ket_tuning fields; the others keep their current values."""
+def get_tuning(precision: int = C128) -> dict:
+    t = Tuning()
+    check(lib().ket_get_tuning_for(int(precision), ctypes.byref(t)))
+    return {f: getattr(t, f) for f in TUNING_FIELDS}
+
+
+def set_tuning(precision: int | None = None, **changes):
+    """Sets the named ket_tuning fields for one precision (C128 or C64) or,
+    with precision None, for both; the other fields keep their values."""
     unknown = set(changes) - set(TUNING_FIELDS)
     if unknown:
         raise TypeError(f"set_tuning: unknown fields {sorted(unknown)}")
-    cur = get_tuning()
-    cur.update(changes)
-    t = Tuning(*(int(cur[f]) for f in TUNING_FIELDS))
-    check(lib().ket_set_tuning(ctypes.byref(t)))
+    for p in ((C128, C64) if precision is None else (precision,)):
+        cur = get_tuning(p)
+        cur.update(changes)
+        t = Tuning(*(int(cur[f]) for f in TUNING_FIELDS))
+        check(lib().ket_set_tuning_for(int(p), ctypes.byref(t)))
 
 
-def get_tuning() -> dict:
-    t = Tuning()
-    check(lib().ket_get_tuning(ctypes.byref(t)))
-    return {f: getattr(t, f) for f in TUNING_FIELDS}
+@contextlib.contextmanager
+def tuning(precision: int | None = None, **changes):
+    """Applies the changes for the duration of a with-block, then restores
+    both precisions' tunings."""
+    saved = {p: get_tuning(p) for p in (C128, C64)}
+    try:
+        set_tuning(precision, **changes)
+        yield
+    finally:
+        for p, t in saved.items():
+            set_tuning(p, **t)
 
 
 def calibrate_copy(nbytes: int = 4 << 30, reps: int = 10, nontemporal: int = 0) -> float:

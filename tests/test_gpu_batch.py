@@ -73,9 +73,10 @@ def test_batch_rejects_bad_input():
 
 @pytest.fixture
 def restore_tuning():
-    t = N.get_tuning()
+    saved = {p: N.get_tuning(p) for p in (N.C128, N.C64)}
     yield
-    N.set_tuning(**t)
+    for p, t in saved.items():
+        N.set_tuning(p, **t)
 
 
 @pytest.mark.parametrize("precision", [N.C128, N.C64])
@@ -89,10 +90,9 @@ def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
     ns = [13, 14, 15, 16, 17, 14, 15] if precision == N.C128 else [14, 15, 16, 17, 15, 16]
     circs = [W.hea(n, 4, seed=900 + i) for i, n in enumerate(ns)]
     obs = [zz_obs(n) for n in ns]
-    t = N.get_tuning()
     out = {}
     for stage in (0, 64):
-        N.set_tuning(**{**t, "block_stage": stage})
+        N.set_tuning(block_stage=stage)
         with N.Batch(ns, precision) as b:
             vals, st = b.run(circs, obs, block_k=k, use_graph=False)
             assert st["instances_hbm"] == len(ns)

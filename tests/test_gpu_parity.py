@@ -52,7 +52,7 @@ def test_each_gate_every_placement_exact(name):
 
 
 @pytest.mark.parametrize("name", GATES)
-def test_pair_exchange_form_exact(name):
+def test_pair_exchange_form_exact(name, restore_tuning):
     """Pair passes with one amplitude per thread (lane-pair swap) give ket's
     values in complex128 and the same bits as the two-amplitude form in
     complex64, at every tested placement."""
@@ -63,22 +63,21 @@ def test_pair_exchange_form_exact(name):
     rng.shuffle(placements)
     placements = placements[:12] + [tuple(range(k)), tuple(range(n - k, n))]
     s0 = random_state(n, 5)
-    t = N.get_tuning()
-    try:
-        for qs in placements:
-            c = Circuit(n)
-            params = list(rng.uniform(0, 2 * np.pi, NPARAMS.get(name, 0)))
-            getattr(c, name)(*qs, *params)
-            want = O.run(c, fuse=False, state=s0)
-            N.set_tuning(**{**t, "pair_exchange": 64})
+    for qs in placements:
+        c = Circuit(n)
+        params = list(rng.uniform(0, 2 * np.pi, NPARAMS.get(name, 0)))
+        getattr(c, name)(*qs, *params)
+        want = O.run(c, fuse=False, state=s0)
+        for nt in (0, 1, 2):
+            N.set_tuning(pair_exchange=64, nontemporal=nt)
             got = gpu_run(c, N.FUSE_NONE, state=s0)
             got64 = gpu_run(c, N.FUSE_NONE, N.C64, state=s0.astype(np.complex64))
-            N.set_tuning(**{**t, "pair_exchange": 0})
+            N.set_tuning(pair_exchange=0, nontemporal=nt)
+            ref = gpu_run(c, N.FUSE_NONE, state=s0)
             ref64 = gpu_run(c, N.FUSE_NONE, N.C64, state=s0.astype(np.complex64))
-            assert np.array_equal(got, want), (name, qs, np.abs(got - want).max())
-            assert np.array_equal(got64, ref64), (name, qs)
-    finally:
-        N.set_tuning(**t)
+            assert np.array_equal(got, want), (name, qs, nt, np.abs(got - want).max())
+            assert np.array_equal(ref, want), (name, qs, nt)
+            assert np.array_equal(got64, ref64), (name, qs, nt)
 
 
 CIRCUITS = {
@@ -150,9 +149,10 @@ STAGED = {
 
 @pytest.fixture
 def restore_tuning():
-    t = N.get_tuning()
+    saved = {p: N.get_tuning(p) for p in (N.C128, N.C64)}
     yield
-    N.set_tuning(**t)
+    for p, t in saved.items():
+        N.set_tuning(p, **t)
 
 
 @pytest.mark.parametrize("cname", sorted(STAGED))
@@ -163,10 +163,9 @@ def test_blocks_lds_tiles_equal_direct(cname, precision, k, restore_tuning):
     same bits as the direct kernels (block_stage 0): same products in the
     same order, only the memory path differs."""
     c = STAGED[cname]()
-    t = N.get_tuning()
-    N.set_tuning(**{**t, "block_stage": 0})
+    N.set_tuning(block_stage=0)
     direct = gpu_run(c, N.FUSE_BLOCKS, precision, k)
-    N.set_tuning(**{**t, "block_stage": 64})
+    N.set_tuning(block_stage=64)
     staged = gpu_run(c, N.FUSE_BLOCKS, precision, k)
     assert np.array_equal(staged, direct), np.abs(staged - direct).max()
     want = O.run(c)
