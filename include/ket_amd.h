@@ -261,7 +261,10 @@ typedef struct ket_tuning {
   int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 1, 2, 4, 8 */
   int32_t nontemporal;      /* gate passes' loads and stores: 0 plain, 1
                                nontemporal, 2 nontemporal when every 128-byte
-                               line the pass writes is written whole */
+                               line the pass writes is written whole, 3 as 2
+                               after widening passes whose condition bits
+                               leave runs under 64 bytes to whole lines
+                               (predicated; the default) */
   int32_t blocks_per_cu;    /* 0: one block per 256*items items; else cap the
                                grid at 256*blocks_per_cu blocks (grid-stride) */
   int32_t block_tiles;      /* MFMA blocks: 16-group tiles each wave loads

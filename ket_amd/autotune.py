@@ -25,7 +25,7 @@ from .gates import Circuit
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-SPACE = {"items_per_thread": [1, 2, 4], "nontemporal": [0, 1, 2], "blocks_per_cu": [0, 32],
+SPACE = {"items_per_thread": [1, 2, 4], "nontemporal": [0, 2, 3], "blocks_per_cu": [0, 32],
          "pair_exchange": [0, 4, 16]}
 
 

@@ -696,8 +696,8 @@ int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
   if (t->items_per_thread != 1 && t->items_per_thread != 2 && t->items_per_thread != 4 &&
       t->items_per_thread != 8)
     return fail(KET_ERR_ARG, "ket_amd: items_per_thread must be 1, 2, 4 or 8");
-  if (t->nontemporal < 0 || t->nontemporal > 2)
-    return fail(KET_ERR_ARG, "ket_amd: nontemporal must be 0, 1 or 2");
+  if (t->nontemporal < 0 || t->nontemporal > 3)
+    return fail(KET_ERR_ARG, "ket_amd: nontemporal must be 0..3");
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
     return fail(KET_ERR_ARG, "ket_amd: blocks_per_cu must be in [0, 64]");
   if (t->block_tiles < 0 || t->block_tiles > 2 || t->block_bpc < 0 || t->block_bpc > 64)

@@ -17,6 +17,7 @@ struct PairSpec {
   int pos[4];
   uint64_t set_mask;
   uint64_t partner;
+  uint64_t pred = 0;  // nonzero: update only where all these bits are set
 };
 
 enum PairOp { OP_DENSE = 0, OP_CH = 1, OP_SWAP = 2 };

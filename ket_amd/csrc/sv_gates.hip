@@ -104,6 +104,10 @@ __global__ __launch_bounds__(kThreads) void k_pair(cplx<T>* __restrict__ s,
         x = b[j];
         y = a[j];
       }
+      if ((lo[j] & sp.pred) != sp.pred) {  // whole-line form: outside the condition
+        x = a[j];
+        y = b[j];
+      }
       st<NT>(s + lo[j], x);
       st<NT>(s + (lo[j] ^ sp.partner), y);
     }
@@ -158,6 +162,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_x(cplx<T>* __restrict__ s, Pa
     } else {  // OP_SWAP
       o = w;
     }
+    if ((lo & sp.pred) != sp.pred) o = v;  // whole-line form: outside the condition
     st<NT>(s + e, o);
   }
 }
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_phase(cplx<T>* __restrict__ s,
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if (idx[j] == ~uint64_t{0}) continue;
-      st<NT>(s + idx[j], cmul<T>(a[j], ph));
+      st<NT>(s + idx[j], (idx[j] & sp.pred) == sp.pred ? cmul<T>(a[j], ph) : a[j]);
     }
   }
 }
@@ -254,11 +259,12 @@ cplx<T> to_c(const double* m, int i) {
 
 // Defaults, per precision (complex128, complex64): the autotuner's picks on
 // MI355X (ket_amd/tuning.json). Gate passes: one pair per thread over a full
-// grid; nontemporal access when every 128-byte line a pass writes is written
-// whole (mode 2); the lane-pair exchange form for partners fewer than 2^4
+// grid; passes whose condition bits leave runs under 64 bytes widened to
+// whole lines, and nontemporal access when every 128-byte line a pass writes
+// is written whole (mode 3); the lane-pair exchange form for partners fewer than 2^4
 // amplitudes away. Fused blocks: 32 workgroups per CU, LDS tiles for
 // complex64 MFMA blocks and for blocks on qubits 0..2.
-Tuning g_tuning[2] = {{1, 2, 0, 1, 32, 3, 4}, {1, 2, 0, 1, 32, 3, 4}};
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 4}, {1, 3, 0, 1, 32, 3, 4}};
 
 // True when every 128-byte line the pass writes is written whole: no
 // condition bit (control, phase bit, the bits of a SWAP) below the line's
@@ -270,7 +276,27 @@ inline bool whole_lines(const PairSpec& sp, bool swap2, int line_bits) {
 }
 
 inline bool use_nt(const Tuning& t, const PairSpec& sp, bool swap2, int line_bits) {
-  return t.nontemporal == 1 || (t.nontemporal == 2 && whole_lines(sp, swap2, line_bits));
+  return t.nontemporal == 1 || (t.nontemporal >= 2 && whole_lines(sp, swap2, line_bits));
+}
+
+// Whole-line form of a pass with condition bits that leave runs shorter
+// than half a line (mode 3): those bits leave the iteration and become a
+// per-amplitude predicate, so every amplitude of a touched run is read and
+// written back (unchanged where the predicate fails), and a pass with no
+// other low bit writes whole lines, nontemporal. The byte model already
+// charges whole lines for such bits. Half-line (64-byte) runs stay as they
+// are: they moved less than whole lines and widening them measured slower.
+inline void widen(const Tuning& t, PairSpec& sp, bool swap2, int line_bits) {
+  const uint64_t low = sp.set_mask & ((uint64_t{1} << (line_bits - 1)) - 1);
+  if (t.nontemporal != 3 || swap2 || !low) return;
+  PairSpec w = sp;
+  w.npos = 0;
+  for (int i = 0; i < sp.npos; ++i)
+    if (!((low >> sp.pos[i]) & 1)) w.pos[w.npos++] = sp.pos[i];
+  w.set_mask = sp.set_mask & ~low;
+  w.n_items = sp.n_items << __builtin_popcountll(low);
+  w.pred = low;
+  sp = w;
 }
 
 inline unsigned tuned_grid(const Tuning& t, uint64_t items, int per_thread) {
@@ -305,10 +331,13 @@ void pair_x_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]
 }
 
 template <class T, int OP>
-void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& sp, const double m[8]) {
+void pair_items(hipStream_t st, cplx<T>* p, const PairSpec& spec, const double m[8]) {
   const Tuning& t = tuning_of<T>();
-  const bool nt = use_nt(t, sp, OP == OP_SWAP && __builtin_popcountll(sp.partner) > 1,
-                         sizeof(T) == 8 ? 3 : 4);
+  const bool swap2 = OP == OP_SWAP && __builtin_popcountll(spec.partner) > 1;
+  const int line_bits = sizeof(T) == 8 ? 3 : 4;
+  PairSpec sp = spec;
+  widen(t, sp, swap2, line_bits);
+  const bool nt = use_nt(t, sp, swap2, line_bits);
   // The exchange form for near partners; with far partners interleaving both
   // halves in one instruction measured slower.
   if (sp.partner && __builtin_ctzll(sp.partner) < t.pair_exchange) {
@@ -342,10 +371,13 @@ void phase_go(hipStream_t st, cplx<T>* p, const PairSpec& sp, cplx<T> ph) {
 }
 
 template <class T>
-hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& sp, cplx<T> ph) {
+hipError_t phase_impl(hipStream_t st, void* s, const PairSpec& spec, cplx<T> ph) {
   auto* p = static_cast<cplx<T>*>(s);
   const Tuning& t = tuning_of<T>();
-  const bool nt = use_nt(t, sp, false, sizeof(T) == 8 ? 3 : 4);
+  const int line_bits = sizeof(T) == 8 ? 3 : 4;
+  PairSpec sp = spec;
+  widen(t, sp, false, line_bits);
+  const bool nt = use_nt(t, sp, false, line_bits);
   switch (t.items_per_thread) {
     case 1: nt ? phase_go<T, 1, 1>(st, p, sp, ph) : phase_go<T, 1, 0>(st, p, sp, ph); break;
     case 2: nt ? phase_go<T, 2, 1>(st, p, sp, ph) : phase_go<T, 2, 0>(st, p, sp, ph); break;

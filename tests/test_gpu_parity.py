@@ -68,7 +68,7 @@ def test_pair_exchange_form_exact(name, restore_tuning):
         params = list(rng.uniform(0, 2 * np.pi, NPARAMS.get(name, 0)))
         getattr(c, name)(*qs, *params)
         want = O.run(c, fuse=False, state=s0)
-        for nt in (0, 1, 2):
+        for nt in (0, 1, 2, 3):
             N.set_tuning(pair_exchange=64, nontemporal=nt)
             got = gpu_run(c, N.FUSE_NONE, state=s0)
             got64 = gpu_run(c, N.FUSE_NONE, N.C64, state=s0.astype(np.complex64))
