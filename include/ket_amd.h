@@ -275,7 +275,7 @@ typedef struct ket_tuning {
                                index order): complex64 blocks of width 4, 5
                                whenever this is > 0, other blocks of width
                                3..5 whose lowest qubit is below it (0: never;
-                               64: always; default 3) */
+                               64: always, the default) */
   int32_t pair_exchange;    /* pair passes whose partner differs in index
                                bits below this run one amplitude per thread,
                                the two lanes of a pair swapping theirs (0:

@@ -262,9 +262,9 @@ cplx<T> to_c(const double* m, int i) {
 // grid; passes whose condition bits leave runs under 64 bytes widened to
 // whole lines, and nontemporal access when every 128-byte line a pass writes
 // is written whole (mode 3); the lane-pair exchange form for partners fewer than 2^4
-// amplitudes away. Fused blocks: 32 workgroups per CU, LDS tiles for
-// complex64 MFMA blocks and for blocks on qubits 0..2.
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 4}, {1, 3, 0, 1, 32, 3, 4}};
+// amplitudes away. Fused blocks: 32 workgroups per CU, LDS tiles (with
+// nontemporal transfers) for every block of width 3..5.
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 64, 4}, {1, 3, 0, 1, 32, 64, 4}};
 
 // True when every 128-byte line the pass writes is written whole: no
 // condition bit (control, phase bit, the bits of a SWAP) below the line's
