@@ -190,11 +190,11 @@ def bench_hea(args):
     for label, prec in (("c128", N.C128), ("c64", N.C64)):
         with N.Batch(ns, prec) as b:
             for w in range(max(args.warmup, 1)):
-                b.run_packed(packed[w % 2], block_k=4, use_graph=True)
+                b.run_packed(packed[w % 2], block_k=0, use_graph=True)
             t0 = time.perf_counter()
             dev_ms, plan_s, pack_s, up_s = 0.0, 0.0, 0.0, 0.0
             for k in range(args.steps):
-                vals, st = b.run_packed(packed[k % 2], block_k=4, use_graph=True)
+                vals, st = b.run_packed(packed[k % 2], block_k=0, use_graph=True)
                 dev_ms += st["device_ms"]
                 plan_s += st["plan_seconds"]
                 pack_s += st["pack_seconds"]
@@ -207,7 +207,8 @@ def bench_hea(args):
                           "upload_s_per_sweep": up_s / args.steps,
                           "alg_GBps_device": st["alg_bytes"] / (st["device_ms"] / 1e3) / 1e9,
                           "launches": st["launches"],
-                          "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"]}
+                          "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"],
+                          "block_k": N.get_tuning(prec)["block_k"] or (4 if prec == N.C128 else 5)}
             log(f"[bench] {label}: {wall:.3f} s/sweep ({dev_ms / args.steps:.1f} ms device, "
                 f"{plan_s / args.steps:.3f} s planning of which {pack_s / args.steps:.3f} s "
                 f"packing, {up_s / args.steps:.3f} s upload), {len(ns) / wall:.0f} instances/s, "
@@ -247,6 +248,11 @@ def bench_brick20(args):
     sv = N.StateVector(20, N.C128)
     w, e, st = time_steps(sv, circ, N.FUSE_REFERENCE, 5, args.steps, args.warmup)
     u = O.uniforms(1, 10_000)
+    for _ in range(max(args.warmup, 1)):
+        # the process's first sample() loads the sampler's kernels
+        sv.set_basis(0)
+        sv.apply_circuit(circ, N.FUSE_REFERENCE)
+        sv.sample(u[:16])
     sv.set_basis(0)
     sv.apply_circuit(circ, N.FUSE_REFERENCE)
     sv.sync()
@@ -285,12 +291,13 @@ def bench_planner(args):
     """Config 5: the learned fusion planner. Stage `train`: time windows of
     5000 generated circuits (28-32 qubits, depth 40, 13-gate set) on the GPU
     as per-gate passes and forced blocks, train the cost model on the GPU and
-    write it to gpurun_out/cost_model.npz. Stage `eval`: 200 held-out
+    write it to --model-out. Stage `eval`: 200 held-out
     circuits of the same distribution run whole with the greedy packer and
     with the learned planner (complex64); wall time of each."""
     from ket_amd import costmodel as CM
     out = {"metric": METRIC}
     states = CM.States(N.C64)
+    mlp = None
     try:
         if args.stage in ("train", "all"):
             circs = CM.config5_circuits(args.train_circuits, seed=1234)
@@ -302,14 +309,16 @@ def bench_planner(args):
             t0 = time.perf_counter()
             mlp, hist = CM.train(X, y, epochs=60, log=log)
             t_train = time.perf_counter() - t0
-            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-            CM.save(mlp, os.path.join(ROOT, "gpurun_out", "cost_model.npz"),
+            dest = os.path.abspath(args.model_out)
+            os.makedirs(os.path.dirname(dest), exist_ok=True)
+            CM.save(mlp, dest,
                     train_passes=len(y), final_loss=hist[-1], epochs=len(hist))
             out["train"] = {"circuits": len(circs), "passes": int(len(y)),
                             "collect_s": t_collect, "train_s": t_train,
                             "loss_first": hist[0], "loss_final": hist[-1]}
         if args.stage in ("eval", "all"):
-            mlp = CM.load()
+            if mlp is None:  # stage eval: the shipped model
+                mlp = CM.load()
             if mlp is None:
                 raise SystemExit("no ket_amd/cost_model.npz: run --stage train first")
             held = CM.config5_circuits(args.eval_circuits, seed=777)
@@ -345,12 +354,16 @@ def main():
     ap.add_argument("--eval-circuits", type=int, default=200)
     ap.add_argument("--ref-subset", type=int, default=20,
                     help="planner eval: circuits also run with ket's own Fuser rule")
+    ap.add_argument("--model-out", default="cost_model_trained.npz",
+                    help="planner train: where the trained cost model is written "
+                         "(the shipped one is ket_amd/cost_model.npz)")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this backend has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); use --gpus 1.",
+              "(a state larger than one GPU's HBM would be split by its top qubits "
+              "across GPUs, which is not built); use --gpus 1.",
               file=sys.stderr)
         return 2
 

@@ -254,11 +254,13 @@ int ket_mlp_train_step(ket_mlp* m, const float* x, const float* t, size_t n,
  * must outlive its use; destroying it unsets it. */
 int ket_mlp_use_for_planning(ket_mlp* m);
 
-/* ---- launch shape of the per-gate passes (the autotuner's search space,
- *      process-wide; no counterpart in the reference, whose CPU loops have
- *      no launch shape) ---- */
+/* ---- launch shape and fusion width (the autotuner's search space, per
+ *      precision, process-wide; no counterpart in the reference, whose CPU
+ *      loops have no launch shape). Defaults: the autotuner's picks on
+ *      MI355X, recorded in ket_amd/tuning.json ---- */
 typedef struct ket_tuning {
-  int32_t items_per_thread; /* amplitude pairs each thread keeps in flight: 1, 2, 4, 8 */
+  int32_t items_per_thread; /* amplitude pairs each thread keeps in flight:
+                               1, 2, 4, 8 (default 1 complex128, 2 complex64) */
   int32_t nontemporal;      /* gate passes' loads and stores: 0 plain, 1
                                nontemporal, 2 nontemporal when every 128-byte
                                line the pass writes is written whole, 3 as 2
@@ -274,12 +276,19 @@ typedef struct ket_tuning {
   int32_t block_stage;      /* LDS tiles for fused blocks (16-byte accesses in
                                index order): complex64 blocks of width 4, 5
                                whenever this is > 0, other blocks of width
-                               3..5 whose lowest qubit is below it (0: never;
-                               64: always, the default) */
+                               3..5 whose lowest qubit is below it and that
+                               carry no phase terms or start inside a 128-byte
+                               line (0: never; default 3 complex128, 64
+                               complex64) */
   int32_t pair_exchange;    /* pair passes whose partner differs in index
                                bits below this run one amplitude per thread,
                                the two lanes of a pair swapping theirs (0:
-                               never; 64: always) */
+                               never; 64: always; default 16 complex128, 4
+                               complex64) */
+  int32_t block_k;          /* fused block width of KET_FUSE_BLOCKS and
+                               KET_FUSE_LEARNED calls (and batches) that pass
+                               block_k = 0: 2..6, or 0 for the built-in 4
+                               (complex128) / 5 (complex64) */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

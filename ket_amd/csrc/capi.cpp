@@ -239,6 +239,12 @@ int ket_sv_create(int n_qubits, int precision, ket_sv** out) {
     return cleanup(hip_fail(e, "hipMalloc(work)"));
   if ((e = hipMalloc(&sv->result, 16 * sizeof(double))) != hipSuccess)
     return cleanup(hip_fail(e, "hipMalloc(result)"));
+  // The sampler's chunk tables (~40 bytes per 4096 amplitudes) and a first
+  // shot buffer (65536 draws), so the first sample() allocates nothing.
+  if ((e = hipMalloc(&sv->sbuf, ketk::sample_buffer_bytes(sv->dim))) != hipSuccess)
+    return cleanup(hip_fail(e, "hipMalloc(sampler)"));
+  if (int st = ensure(&sv->io, &sv->io_cap, size_t{65536} * 16, "hipMalloc(io)"))
+    return cleanup(st);
   for (auto& ev : sv->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return cleanup(hip_fail(e, "hipEventCreate"));
@@ -706,9 +712,11 @@ int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
     return fail(KET_ERR_ARG, "ket_amd: block_stage must be in [0, 64]");
   if (t->pair_exchange < 0 || t->pair_exchange > 64)
     return fail(KET_ERR_ARG, "ket_amd: pair_exchange must be in [0, 64]");
+  if (t->block_k != 0 && (t->block_k < 2 || t->block_k > 6))
+    return fail(KET_ERR_ARG, "ket_amd: block_k must be 0 or in [2, 6]");
   *out = {t->items_per_thread, t->nontemporal, t->blocks_per_cu,
           t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
-          t->block_stage, t->pair_exchange};
+          t->block_stage, t->pair_exchange, t->block_k};
   return KET_OK;
 }
 
@@ -720,6 +728,7 @@ void fill_tuning(const ketk::Tuning& k, ket_tuning* t) {
   t->block_bpc = k.block_bpc;
   t->block_stage = k.block_stage;
   t->pair_exchange = k.pair_exchange;
+  t->block_k = k.block_k;
 }
 }  // namespace
 

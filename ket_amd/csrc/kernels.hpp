@@ -35,6 +35,8 @@ struct Tuning {
   int block_stage;   // blocks whose lowest qubit is below this go through LDS tiles
   int pair_exchange; // pair passes whose partner's lowest index bit is below
                      // this: one amplitude per thread, lane-pair swap
+  int block_k;       // fused block width when a call passes 0 (0: 4 for
+                     // complex128, 5 for complex64)
 };
 // Per precision (0 = complex128, 1 = complex64).
 void set_tuning(int precision, const Tuning& t);

@@ -49,9 +49,14 @@ struct Plan {
 // Width of a 128-byte line in index bits: 3 for complex128, 4 for complex64.
 inline int line_bits(int precision) { return precision == 0 ? 3 : 4; }
 
-// Block width when the caller passes 0: complex128 blocks of 5 qubits are
-// bound by FP64 FMA throughput rather than HBM, complex64 ones run on MFMA.
-inline int default_block_k(int precision) { return precision == 0 ? 4 : 5; }
+// Block width when the caller passes 0: the tuned width if one is set
+// (ket_tuning.block_k), else 4 for complex128 (5-qubit FP64 blocks are bound
+// by FMA throughput rather than HBM) and 5 for complex64 (MFMA).
+inline int default_block_k(int precision) {
+  const int t = ketk::get_tuning(precision).block_k;
+  if (t) return t;
+  return precision == 0 ? 4 : 5;
+}
 
 // Lowest qubit of a MatrixExact or Block op.
 inline int lowest_qubit(const Op& o) {

@@ -257,14 +257,17 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
-// Defaults, per precision (complex128, complex64): the autotuner's picks on
-// MI355X (ket_amd/tuning.json). Gate passes: one pair per thread over a full
-// grid; passes whose condition bits leave runs under 64 bytes widened to
-// whole lines, and nontemporal access when every 128-byte line a pass writes
-// is written whole (mode 3); the lane-pair exchange form for partners fewer than 2^4
-// amplitudes away. Fused blocks: 32 workgroups per CU, LDS tiles (with
-// nontemporal transfers) for every block of width 3..5.
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 64, 4}, {1, 3, 0, 1, 32, 64, 4}};
+// Defaults, per precision: the autotuner's picks on MI355X
+// (ket_amd/tuning.json). Gate passes: one pair per thread (complex128) or two
+// (complex64) over a full grid; passes whose condition bits leave runs under
+// 64 bytes widened to whole lines, and nontemporal access when every 128-byte
+// line a pass writes is written whole (mode 3); the lane-pair exchange form
+// for partners fewer than 2^16 (complex128) or 2^4 (complex64) amplitudes
+// away.
+// Fused blocks: 32 workgroups per CU; complex64 blocks always through LDS
+// tiles, complex128 ones when their lowest qubit is 0..2 (and the launch rule
+// of launch_block allows), with nontemporal tile transfers.
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 64, 16, 0}, {2, 3, 0, 1, 32, 64, 4, 0}};
 
 // True when every 128-byte line the pass writes is written whole: no
 // condition bit (control, phase bit, the bits of a SWAP) below the line's

@@ -146,11 +146,11 @@ class Tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("block_tiles", ctypes.c_int32),
                 ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32),
-                ("pair_exchange", ctypes.c_int32)]
+                ("pair_exchange", ctypes.c_int32), ("block_k", ctypes.c_int32)]
 
 
 TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tiles", "block_bpc",
-                 "block_stage", "pair_exchange")
+                 "block_stage", "pair_exchange", "block_k")
 
 
 def get_tuning(precision: int = C128) -> dict:
@@ -418,12 +418,12 @@ class Batch:
         toff = np.ascontiguousarray(toff, np.uint64)
         return gates, goff, zm, cf, toff
 
-    def run(self, circuits, observables, block_k: int = 4, use_graph: bool = True):
+    def run(self, circuits, observables, block_k: int = 0, use_graph: bool = True):
         """circuits[i]: ket_amd.gates.Circuit; observables[i]: list of
         (coeff, pauli) with only I/Z characters. Returns (values, stats)."""
         return self.run_packed(self.pack(circuits, observables), block_k, use_graph)
 
-    def run_packed(self, packed, block_k: int = 4, use_graph: bool = True):
+    def run_packed(self, packed, block_k: int = 0, use_graph: bool = True):
         gates, goff, zm, cf, toff = packed
         out = np.zeros(len(goff) - 1, np.float64)
         st = BatchStats()

@@ -37,3 +37,32 @@ def test_gate_record_layout_matches_header():
     assert "int32_t qubits[3];" in text and "double params[3];" in text
     assert GATE_DTYPE.itemsize == 48
     assert GATE_DTYPE.fields["p"][1] == 24
+
+
+def test_tuning_struct_matches_header():
+    text = open(os.path.join(ROOT, "include", "ket_amd.h")).read()
+    body = re.search(r"typedef struct ket_tuning \{(.*?)\} ket_tuning;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"int32_t\s+(\w+);", body)
+    assert tuple(fields) == N.TUNING_FIELDS
+    assert ctypes.sizeof(N.Tuning) == 4 * len(fields)
+
+
+def test_tuning_round_trip_and_validation():
+    """ket_set_tuning_for / ket_get_tuning_for keep each precision's fields
+    and reject out-of-range values (no device call)."""
+    import pytest
+    saved = {p: N.get_tuning(p) for p in (N.C128, N.C64)}
+    try:
+        N.set_tuning(N.C64, block_k=3, pair_exchange=16)
+        assert N.get_tuning(N.C64)["block_k"] == 3
+        assert N.get_tuning(N.C64)["pair_exchange"] == 16
+        assert N.get_tuning(N.C128) == saved[N.C128]
+        for bad in ({"block_k": 1}, {"block_k": 7}, {"nontemporal": 4}, {"items_per_thread": 3},
+                    {"block_stage": 65}, {"pair_exchange": -1}):
+            with pytest.raises(ValueError):
+                N.set_tuning(N.C128, **bad)
+        assert N.get_tuning(N.C128) == saved[N.C128]
+    finally:
+        for p, t in saved.items():
+            N.set_tuning(p, **t)

@@ -299,3 +299,22 @@ def test_errors_are_statuses():
         N.StateVector(40)
     with pytest.raises(ValueError):
         N.StateVector(41)
+
+
+@pytest.mark.parametrize("precision", [N.C128, N.C64])
+def test_tuned_block_width_applies_when_call_passes_zero(precision, restore_tuning):
+    """ket_tuning.block_k is the fusion width of calls that pass block_k = 0,
+    and the result equals the same width passed explicitly."""
+    c = W.brickwork(12, 6, seed=21, measure=False)
+    with N.StateVector(12, precision) as sv:
+        for k in (3, 4, 5):
+            N.set_tuning(precision, block_k=k)
+            recs = sv.profile_circuit(c, N.FUSE_BLOCKS_ALL, 0)
+            ks = [r["k"] for r in recs if r["kind"] == "block"]
+            assert ks and max(ks) == k
+            sv.set_basis(0)
+            sv.apply_circuit(c, N.FUSE_BLOCKS, 0)
+            auto = sv.to_numpy()
+            sv.set_basis(0)
+            sv.apply_circuit(c, N.FUSE_BLOCKS, k)
+            assert np.array_equal(auto, sv.to_numpy())
