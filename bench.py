@@ -249,10 +249,11 @@ def bench_brick20(args):
     w, e, st = time_steps(sv, circ, N.FUSE_REFERENCE, 5, args.steps, args.warmup)
     u = O.uniforms(1, 10_000)
     for _ in range(max(args.warmup, 1)):
-        # the process's first sample() loads the sampler's kernels
+        # the process's first sample() loads the sampler's kernels and sets up
+        # the runtime's staging for host copies of this size
         sv.set_basis(0)
         sv.apply_circuit(circ, N.FUSE_REFERENCE)
-        sv.sample(u[:16])
+        sv.sample(u)
     sv.set_basis(0)
     sv.apply_circuit(circ, N.FUSE_REFERENCE)
     sv.sync()

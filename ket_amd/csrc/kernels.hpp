@@ -59,6 +59,34 @@ hipError_t launch_matrix_exact(hipStream_t st, int precision, void* s,
 hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
                           int src_precision, const void* src, uint64_t count);
 
+// Exact runs: consecutive per-gate passes applied to LDS tiles of 2^TB
+// amplitudes (TB = run_tile_bits), each with its own kernel's arithmetic, in
+// order, so a run equals its passes bit for bit. A record is a RunRecord,
+// then n_ops RunOps, then the 4^k complex128 matrices of its exact-matrix
+// ops. Tile positions P (pmask) hold every bit an op moves; local bit j of a
+// tile index is bit j-th-lowest of P.
+enum RunKind { RUN_DENSE = 0, RUN_CH = 1, RUN_SWAP = 2, RUN_PHASE = 3, RUN_MATRIX = 4 };
+struct RunOp {
+  int32_t kind;
+  int32_t npos;        // local positions in pos (matrix: its width k)
+  int32_t pos[4];      // local positions, ascending (zero bits inserted there)
+  int32_t mpos[4];     // matrix ops: local position of matrix index bit b
+  uint32_t set;        // local bits set in the lower index of a pair / phase
+  uint32_t partner;    // local bits a pair's two indices differ in
+  uint64_t out_mask;   // bits outside P that must all be set (outside controls)
+  double m[8];         // 2x2 row-major (re, im); CH: m[0]; phase: m[0], m[1]
+  uint32_t mat_offset; // matrix ops: bytes from the record start
+  int32_t pad;
+};
+struct RunRecord {
+  uint64_t pmask;
+  int32_t n_ops;
+  int32_t pad;
+};
+inline int run_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
+// n >= run_tile_bits(precision); rec_dev: the record in device memory.
+hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev);
+
 // Pool record of a fused block (see sv_block.hip). Matrix bit b of an
 // element index is qubit qubits[b]; offsets[e] = that element's index bits.
 // Trailing phase terms are sorted by the block-local bit they depend on:

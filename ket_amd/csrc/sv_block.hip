@@ -886,6 +886,126 @@ hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uin
   return hipGetLastError();
 }
 
+// ---- exact runs: consecutive per-gate passes on one LDS tile ----
+// Each op repeats its per-gate kernel's arithmetic (k_pair, k_phase,
+// k_matrix_exact in sv_gates.hip) on the tile's amplitudes; ops run in plan
+// order with a barrier between them, so every amplitude sees the same
+// operations on the same values as with one pass per op.
+
+// One exact-matrix op of width K: per group, acc = 0; acc += M[r][c] * in[c]
+// for c ascending (the matrix converted to T element by element, as
+// k_matrix_exact does).
+template <class T, int TB, int K>
+__device__ __forceinline__ void run_matrix(ketd::cplx<T>* L, const RunOp& op,
+                                           const unsigned char* rec) {
+  constexpr int D = 1 << K;
+  const auto* mat = reinterpret_cast<const double2*>(rec + op.mat_offset);
+  uint32_t off[D];
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+      if ((e >> b) & 1) o |= 1u << op.mpos[b];
+    off[e] = o;
+  }
+  for (uint32_t g = threadIdx.x; g < (1u << (TB - K)); g += kThreads) {
+    const uint32_t base = (uint32_t)ketd::insert_zeros(g, K, op.pos);
+    ketd::cplx<T> in[D];
+#pragma unroll
+    for (int e = 0; e < D; ++e) in[e] = L[base | off[e]];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      ketd::cplx<T> acc = ketd::mk<T>(T(0), T(0));
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        acc = ketd::cadd<T>(acc, ketd::cmul<T>(ketd::mk<T>((T)mat[r * D + c].x,
+                                                           (T)mat[r * D + c].y),
+                                               in[c]));
+      L[base | off[r]] = acc;
+    }
+  }
+}
+
+template <class T, int TB>
+__device__ __forceinline__ void run_op(ketd::cplx<T>* L, const RunOp& op,
+                                       const unsigned char* rec) {
+  switch (op.kind) {
+    case RUN_DENSE:
+    case RUN_CH:
+    case RUN_SWAP: {
+      const ketd::cplx<T> m00 = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
+      const ketd::cplx<T> m01 = ketd::mk<T>((T)op.m[2], (T)op.m[3]);
+      const ketd::cplx<T> m10 = ketd::mk<T>((T)op.m[4], (T)op.m[5]);
+      const ketd::cplx<T> m11 = ketd::mk<T>((T)op.m[6], (T)op.m[7]);
+      for (uint32_t it = threadIdx.x; it < ((1u << TB) >> op.npos); it += kThreads) {
+        const uint32_t lo = (uint32_t)ketd::insert_zeros(it, op.npos, op.pos) | op.set;
+        const uint32_t hi = lo ^ op.partner;
+        const ketd::cplx<T> a = L[lo], b = L[hi];
+        ketd::cplx<T> x, y;
+        if (op.kind == RUN_DENSE) {
+          x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
+          y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
+        } else if (op.kind == RUN_CH) {
+          const T r = m00.x;
+          x = ketd::mk<T>((a.x + b.x) * r, (a.y + b.y) * r);
+          y = ketd::mk<T>((a.x - b.x) * r, (a.y - b.y) * r);
+        } else {
+          x = b;
+          y = a;
+        }
+        L[lo] = x;
+        L[hi] = y;
+      }
+      break;
+    }
+    case RUN_PHASE: {
+      const ketd::cplx<T> ph = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
+      for (uint32_t it = threadIdx.x; it < ((1u << TB) >> op.npos); it += kThreads) {
+        const uint32_t i = (uint32_t)ketd::insert_zeros(it, op.npos, op.pos) | op.set;
+        L[i] = ketd::cmul<T>(L[i], ph);
+      }
+      break;
+    }
+    case RUN_MATRIX:
+      if (op.npos == 1) run_matrix<T, TB, 1>(L, op, rec);
+      else if (op.npos == 2) run_matrix<T, TB, 2>(L, op, rec);
+      else run_matrix<T, TB, 3>(L, op, rec);
+      break;
+    default:
+      break;
+  }
+}
+
+// One workgroup per tile (grid-stride): load the tile in index order, apply
+// the ops whose outside controls the tile satisfies, store it back.
+template <class T, int TB>
+__global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict__ s,
+                                                        uint64_t n_tiles,
+                                                        const unsigned char* __restrict__ rec) {
+  using IO = TileIO<T, TB, kThreads>;
+  __shared__ ketd::cplx<T> L[1 << TB];
+  const RunRecord& h = *reinterpret_cast<const RunRecord*>(rec);
+  const auto* ops = reinterpret_cast<const RunOp*>(rec + sizeof(RunRecord));
+  TileMap mp{};
+  mp.pmask = h.pmask;
+  mp.nloc = (1u << TB) - 1;  // no swizzle: LDS slot = tile index
+  IO io;
+  io.init(mp, threadIdx.x);
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint64_t fixed = deposit_outside(tile, h.pmask);
+    io.load(s, fixed, L);
+    for (int o = 0; o < h.n_ops; ++o) {
+      __syncthreads();
+      if ((fixed & ops[o].out_mask) != ops[o].out_mask) continue;  // uniform
+      run_op<T, TB>(L, ops[o], rec);
+    }
+    __syncthreads();
+    io.store(s, fixed, L);
+    __syncthreads();
+  }
+}
+
 // The launch rule for vector blocks (see launch_block).
 bool vector_tiles(const Tuning& t, int precision, int k, int n_terms, int lowest) {
   const int line_bits = precision == 0 ? 3 : 4;
@@ -951,6 +1071,22 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
               : block_vec<double, false>(st, s, groups, k, rec);
   return ph ? block_vec<float, true>(st, s, groups, k, rec)
             : block_vec<float, false>(st, s, groups, k, rec);
+}
+
+hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev) {
+  const int tb = run_tile_bits(precision);
+  if (n < tb) return hipErrorInvalidValue;
+  const uint64_t n_tiles = uint64_t{1} << (n - tb);
+  const Tuning t = get_tuning(precision);
+  uint64_t g = n_tiles;
+  const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
+  if (g > cap) g = cap;
+  const auto* rec = static_cast<const unsigned char*>(rec_dev);
+  if (precision == 0)
+    k_run_tile<double, 11><<<(unsigned)g, kThreads, 0, st>>>(static_cast<double2*>(s), n_tiles, rec);
+  else
+    k_run_tile<float, 12><<<(unsigned)g, kThreads, 0, st>>>(static_cast<float2*>(s), n_tiles, rec);
+  return hipGetLastError();
 }
 
 }  // namespace ketk
