@@ -6,6 +6,9 @@ Workload (default): the 30-qubit QFT preceded by one random U layer
 kernels. `value` is gates per second of that form.
 
 Besides the timed steps the run reports:
+  * the same circuit as exact runs (consecutive per-gate passes applied on
+    LDS tiles, equal to the plain form bit for bit) and with ket's Fuser
+    rule plus exact runs (the drop-in default, equal to ket::run);
   * the same circuit with fused dense blocks (complex128, k = 4) and in
     complex64 with fused blocks (k = 4, 5), with 1-F and max |amplitude error|
     of the complex64 state against the plain complex128 state, both resident;
@@ -163,8 +166,20 @@ def bench_brick33(args, circ, desc):
                              "blocks": st["blocks"], "one_minus_F": 1 - f, "max_abs_err": dmax,
                              "speedup_vs_plain": wp / ps_f}},
         "device": N.device_info()["name"]})
-    plain.close()
     fused.close()
+    # Exact runs (per-gate arithmetic on LDS tiles) on a second complex128
+    # state: equal to the plain form bit for bit.
+    exact = N.StateVector(n, N.C128)
+    log("[bench] exact runs complex128 (one step)")
+    wx, ex, stx = time_steps(exact, circ, N.FUSE_EXACT, 0, 1, 0)
+    fx, dx = exact.fidelity(plain)
+    log(f"[bench] exact runs: {wx:.2f} s, {gates / wx:.1f} gates/s, {stx['launches']} launches, "
+        f"x{wp / wx:.2f} vs plain, max|d| vs plain {dx:.1e}")
+    out["exact_runs"] = {"c128": {"s_per_step": wx, "gates_per_s": gates / wx,
+                                  "launches": stx["launches"], "max_abs_diff_vs_plain": dx,
+                                  "speedup_vs_plain": wp / wx}}
+    exact.close()
+    plain.close()
     if not args.no_cpu:
         # The reference on the real 2^33 shape (128 GiB of host memory): the
         # first 2 and first 6 gates of the same circuit.
@@ -328,7 +343,8 @@ def bench_planner(args):
             out["eval"] = res
             log(f"[bench] greedy {res['greedy_ms']:.0f} ms, learned {res['learned_ms']:.0f} ms, "
                 f"speedup {res['speedup']:.3f}; vs ket's Fuser on {res.get('reference_fuser_circuits')} "
-                f"circuits: {res.get('speedup_vs_reference_fuser')}")
+                f"circuits: {res.get('speedup_vs_reference_fuser')} (one pass per group), "
+                f"{res.get('speedup_vs_reference_fuser_runs')} (exact runs)")
     finally:
         states.close()
     ev = out.get("eval", {})
@@ -416,9 +432,15 @@ def main():
         plain.set_basis(0)
         plain.apply_circuit(circ, N.FUSE_NONE)
         plain.sync()
-        for label, prec, k in [("c128_k4", N.C128, 4), ("c64_k4", N.C64, 4), ("c64_k5", N.C64, 5)]:
+        # exact runs: per-gate arithmetic on LDS tiles (equal to the plain
+        # form bit for bit), and ket's Fuser with runs (the drop-in default)
+        for label, prec, fusion, k in [("c128_exact_runs", N.C128, N.FUSE_EXACT, 0),
+                                       ("c128_ket_fuser_runs", N.C128, N.FUSE_REFERENCE, 0),
+                                       ("c128_k4", N.C128, N.FUSE_BLOCKS, 4),
+                                       ("c64_k4", N.C64, N.FUSE_BLOCKS, 4),
+                                       ("c64_k5", N.C64, N.FUSE_BLOCKS, 5)]:
             sv = N.StateVector(n, prec)
-            w, e, st = time_steps(sv, circ, N.FUSE_BLOCKS, k, args.steps, args.warmup)
+            w, e, st = time_steps(sv, circ, fusion, k, args.steps, args.warmup)
             f, dmax = sv.fidelity(plain)
             ps = w / args.steps
             fused[label] = {"ms_per_step": ps * 1e3, "gates_per_s": gates / ps,

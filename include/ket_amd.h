@@ -66,13 +66,19 @@ typedef enum ket_fusion {
   KET_FUSE_NONE = 0,      /* one launch per gate (ket's Stepper/apply_node) */
   KET_FUSE_REFERENCE = 1, /* ket's greedy same-qubit-set Fuser
                              (src/simulator.cpp:400-477); with KET_C128 the
-                             result equals ket::run's value for value */
+                             result equals ket::run's value for value.
+                             Consecutive passes run as exact runs on LDS
+                             tiles unless ket_tuning.exact_runs is 0 (the
+                             values do not change) */
   KET_FUSE_BLOCKS = 2,    /* k-qubit packer: dense 2^k blocks (k = block_k) */
   KET_FUSE_LEARNED = 3,   /* packer whose block width and block-or-not
                              choices a trained cost model picks; without a
                              model it is KET_FUSE_BLOCKS */
-  KET_FUSE_BLOCKS_ALL = 4 /* the packer with every group emitted as a block
+  KET_FUSE_BLOCKS_ALL = 4, /* the packer with every group emitted as a block
                              (cost-model training data, batched sweeps) */
+  KET_FUSE_EXACT = 5      /* KET_FUSE_NONE's per-gate arithmetic with
+                             consecutive passes grouped into exact runs on
+                             LDS tiles: equal to KET_FUSE_NONE bit for bit */
 } ket_fusion;
 
 typedef struct ket_sv ket_sv; /* a state vector resident on one GPU */
@@ -202,7 +208,9 @@ typedef enum ket_launch_kind {
   KET_LAUNCH_PERMUTE = 2,  /* X / CX / CCX / SWAP / CSWAP exchanges */
   KET_LAUNCH_PHASE = 3,    /* diagonal phase on a bit mask */
   KET_LAUNCH_MATRIX = 4,   /* ket's exact apply_matrix arithmetic */
-  KET_LAUNCH_BLOCK = 5     /* fused dense 2^k block */
+  KET_LAUNCH_BLOCK = 5,    /* fused dense 2^k block */
+  KET_LAUNCH_RUN = 6       /* exact run: consecutive per-gate passes on LDS
+                              tiles (k = the number of passes) */
 } ket_launch_kind;
 
 #define KET_MLP_INPUTS 16
@@ -289,6 +297,12 @@ typedef struct ket_tuning {
                                KET_FUSE_LEARNED calls (and batches) that pass
                                block_k = 0: 2..6, or 0 for the built-in 4
                                (complex128) / 5 (complex64) */
+  int32_t exact_runs;       /* KET_FUSE_REFERENCE groups consecutive passes
+                               into exact runs on LDS tiles: 1 (default) or
+                               0 (one launch per pass) */
+  int32_t run_low_bits;     /* exact runs: the lowest index bits every tile
+                               keeps, so its HBM rows are 2^run_low_bits
+                               amplitudes long (3..10; 0: the default) */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

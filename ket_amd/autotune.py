@@ -12,7 +12,9 @@ wall clocks (whole runs), keeping the fastest point:
 * fusion width: whole runs of a brickwork and a QFT circuit and a batched
   sweep of hardware-efficient-ansatz instances of uneven widths, for block
   widths 3, 4 and 5; the score is the geometric mean over the three of the
-  time relative to the best width for that workload.
+  time relative to the best width for that workload;
+* exact runs: whole KET_FUSE_EXACT runs of a brickwork, a QFT and a random
+  13-gate-set circuit for tile low-bit counts 3..6, scored the same way.
 
 Results are stored per precision in a JSON table that `apply_table` loads.
 
@@ -199,6 +201,39 @@ def tune_width(n: int = 30, precision: int = N.C128, reps: int = 3, instances: i
             "all": rows}
 
 
+RUN_LOWS = [3, 4, 5, 6]
+
+
+def tune_runs(n: int = 30, precision: int = N.C128, reps: int = 3, log=print) -> dict:
+    """Whole-run wall time of KET_FUSE_EXACT (consecutive per-gate passes on
+    LDS tiles) for each number of low bits the tiles keep."""
+    circs = {"brick": W.brickwork(n, 6, seed=3, measure=False), "qft": W.qft_u(n, seed=3),
+             "mixed": W.random_mixed(n, 12, seed=3)}
+    rows = []
+    with N.StateVector(n, precision) as sv:
+        for low in RUN_LOWS:
+            N.set_tuning(precision, run_low_bits=low)
+            row = {"run_low_bits": low}
+            for name, c in circs.items():
+                rec = c.records()
+
+                def run(rec=rec):
+                    sv.set_basis(0)
+                    sv.apply_records(rec, N.FUSE_EXACT, 0)
+                    sv.sync()
+                row[name + "_ms"] = _median_wall(run, reps) * 1e3
+            rows.append(row)
+            log(f"[tune] runs low={low}: " + " ".join(f"{kk}={v:.1f}" for kk, v in row.items()
+                                                       if kk != "run_low_bits"))
+    keys = [kk for kk in rows[0] if kk.endswith("_ms")]
+    best_of = {kk: min(r[kk] for r in rows) for kk in keys}
+    for r in rows:
+        r["score"] = math.exp(sum(math.log(r[kk] / best_of[kk]) for kk in keys) / len(keys))
+    best = min(rows, key=lambda r: r["score"])
+    N.set_tuning(precision, run_low_bits=best["run_low_bits"])
+    return {"qubits": n, "best": {"run_low_bits": best["run_low_bits"]}, "all": rows}
+
+
 def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
     """Set the tuning recorded for this precision (gate passes, fused blocks
     and fusion width), if the table has it; returns the fields set."""
@@ -208,7 +243,8 @@ def apply_table(path: str = TABLE, precision: int = N.C128) -> dict | None:
         table = json.load(f)
     key = "c128" if precision == N.C128 else "c64"
     applied = {}
-    for entry in (table.get(key), table.get(key + "_blocks"), table.get(key + "_width")):
+    for entry in (table.get(key), table.get(key + "_blocks"), table.get(key + "_width"),
+                  table.get(key + "_runs")):
         if entry:
             applied.update({k: v for k, v in entry["best"].items() if k in N.TUNING_FIELDS})
     if not applied:
@@ -237,6 +273,8 @@ def main():
         log(f"[tune] {key} blocks best {table[key + '_blocks']['best']}")
         table[key + "_width"] = tune_width(args.qubits, prec, args.reps, args.instances, log)
         log(f"[tune] {key} width best {table[key + '_width']['best']}")
+        table[key + "_runs"] = tune_runs(args.qubits, prec, args.reps, log)
+        log(f"[tune] {key} runs best {table[key + '_runs']['best']}")
     out = json.dumps(table, indent=1)
     if args.out:
         with open(args.out, "w") as f:

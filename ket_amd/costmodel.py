@@ -122,15 +122,21 @@ def evaluate(mlp: N.MLP, circuits, states: States, log=None, ref_subset: int = 0
     """Learned planner vs the greedy packer (k = 5, bytes rule) on whole
     circuits, the model's relative error on the learned plans' passes, and,
     on the first `ref_subset` circuits, learned vs ket's own greedy Fuser
-    (KET_FUSE_REFERENCE, the reference's fusion rule)."""
-    tg, tl, errs, tr, tlr = [], [], [], [], []
+    (KET_FUSE_REFERENCE, the reference's fusion rule), run as the reference
+    runs it (one pass per fused group) and as exact runs on LDS tiles."""
+    tg, tl, errs, tr, trr, tlr = [], [], [], [], [], []
     for i, c in enumerate(circuits):
         sv = states.get(c.n_qubits)
         mlp.use_for_planning(False)
         time_circuit(sv, c, N.FUSE_BLOCKS, 5)  # warm
         tg.append(time_circuit(sv, c, N.FUSE_BLOCKS, 5))
         if i < ref_subset:
-            tr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
+            with N.tuning(sv.precision, exact_runs=0):
+                time_circuit(sv, c, N.FUSE_REFERENCE, 5)  # warm
+                tr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
+            with N.tuning(sv.precision, exact_runs=1):
+                time_circuit(sv, c, N.FUSE_REFERENCE, 5)
+                trr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
         mlp.use_for_planning(True)
         tl.append(time_circuit(sv, c, N.FUSE_LEARNED, 5))
         if i < ref_subset:
@@ -153,8 +159,10 @@ def evaluate(mlp: N.MLP, circuits, states: States, log=None, ref_subset: int = 0
            "model_total_time_rel_error_median": float(np.median(errs))}
     if tr:
         out.update({"reference_fuser_circuits": len(tr), "reference_fuser_ms": float(sum(tr)),
+                    "reference_fuser_runs_ms": float(sum(trr)),
                     "learned_ms_same_circuits": float(sum(tlr)),
-                    "speedup_vs_reference_fuser": float(sum(tr) / sum(tlr))})
+                    "speedup_vs_reference_fuser": float(sum(tr) / sum(tlr)),
+                    "speedup_vs_reference_fuser_runs": float(sum(trr) / sum(tlr))})
     return out
 
 

@@ -132,6 +132,9 @@ int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
       e = ketk::launch_block(sv->stream, sv->precision, sv->data, sv->n, o.k,
                              o.n_terms, pool_dev + o.pool_offset, ketplan::lowest_qubit(o));
       break;
+    case ketplan::Kind::Run:
+      e = ketk::launch_run(sv->stream, sv->precision, sv->data, sv->n, pool_dev + o.pool_offset);
+      break;
   }
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return KET_OK;
@@ -544,6 +547,11 @@ int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, in
         r.k = o.k;
         for (int b = 0; b < o.k; ++b) r.qubit_mask |= uint64_t{1} << o.qubits[b];
         break;
+      case ketplan::Kind::Run:
+        r.kind = KET_LAUNCH_RUN;
+        r.k = o.run_ops;
+        r.qubit_mask = o.run_pmask;
+        break;
     }
     r.n_gates = o.n_gates;
     r.n_terms = o.n_terms;
@@ -714,9 +722,13 @@ int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
     return fail(KET_ERR_ARG, "ket_amd: pair_exchange must be in [0, 64]");
   if (t->block_k != 0 && (t->block_k < 2 || t->block_k > 6))
     return fail(KET_ERR_ARG, "ket_amd: block_k must be 0 or in [2, 6]");
+  if (t->exact_runs != 0 && t->exact_runs != 1)
+    return fail(KET_ERR_ARG, "ket_amd: exact_runs must be 0 or 1");
+  if (t->run_low_bits != 0 && (t->run_low_bits < 3 || t->run_low_bits > 10))
+    return fail(KET_ERR_ARG, "ket_amd: run_low_bits must be 0 or in [3, 10]");
   *out = {t->items_per_thread, t->nontemporal, t->blocks_per_cu,
           t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
-          t->block_stage, t->pair_exchange, t->block_k};
+          t->block_stage, t->pair_exchange, t->block_k, t->exact_runs, t->run_low_bits};
   return KET_OK;
 }
 
@@ -729,6 +741,8 @@ void fill_tuning(const ketk::Tuning& k, ket_tuning* t) {
   t->block_stage = k.block_stage;
   t->pair_exchange = k.pair_exchange;
   t->block_k = k.block_k;
+  t->exact_runs = k.exact_runs;
+  t->run_low_bits = k.run_low_bits;
 }
 }  // namespace
 

@@ -37,6 +37,8 @@ struct Tuning {
                      // this: one amplitude per thread, lane-pair swap
   int block_k;       // fused block width when a call passes 0 (0: 4 for
                      // complex128, 5 for complex64)
+  int exact_runs;    // KET_FUSE_REFERENCE: group passes into exact runs
+  int run_low_bits;  // exact runs: low index bits kept in every tile
 };
 // Per precision (0 = complex128, 1 = complex64).
 void set_tuning(int precision, const Tuning& t);
@@ -83,6 +85,7 @@ struct RunRecord {
   int32_t n_ops;
   int32_t pad;
 };
+static_assert(sizeof(RunOp) == 128 && sizeof(RunRecord) == 16, "run record layout");
 inline int run_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 // n >= run_tile_bits(precision); rec_dev: the record in device memory.
 hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev);

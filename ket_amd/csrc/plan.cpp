@@ -222,6 +222,170 @@ Op op_for_matrix(Plan& plan, int n, int precision, int k, const int* qubits,
 
 namespace {
 
+// Bits a pass moves amplitudes across (pairs: the bits its two indices
+// differ in; exact matrices: their qubits); phases move none. -1: not an
+// exact per-gate pass.
+int64_t moved_bits(const Op& o) {
+  switch (o.kind) {
+    case Kind::Pair:
+      return (int64_t)o.spec.partner;
+    case Kind::Phase:
+      return 0;
+    case Kind::MatrixExact: {
+      if (o.k > 3) return -1;
+      uint64_t m = 0;
+      for (int b = 0; b < o.k; ++b) m |= bit(o.qubits[b]);
+      return (int64_t)m;
+    }
+    default:
+      return -1;
+  }
+}
+
+// Index of bit position p among the set bits of mask (p must be set).
+int rank_in(uint64_t mask, int p) { return __builtin_popcountll(mask & (bit(p) - 1)); }
+
+// Bits of x at the positions of mask, packed low.
+uint32_t compress(uint64_t x, uint64_t mask) {
+  uint32_t r = 0;
+  int j = 0;
+  for (uint64_t m = mask; m; m &= m - 1, ++j)
+    if (x & m & (~m + 1)) r |= 1u << j;
+  return r;
+}
+
+// P: the moved bits plus the lowest other bits, tb in all.
+uint64_t tile_positions(uint64_t moved, int tb) {
+  uint64_t p = moved;
+  int need = tb - __builtin_popcountll(moved);
+  for (int q = 0; need > 0; ++q)
+    if (!(p & bit(q))) {
+      p |= bit(q);
+      --need;
+    }
+  return p;
+}
+
+// The local form of one pass on tile positions P; false if a condition bit
+// outside P would have to be clear (never for ket's gates).
+bool local_op(const Plan& plan, const Op& o, uint64_t P, ketk::RunOp* r,
+              std::vector<unsigned char>* mats) {
+  *r = ketk::RunOp{};
+  if (o.kind == Kind::MatrixExact) {
+    r->kind = ketk::RUN_MATRIX;
+    r->npos = o.k;
+    std::vector<int> loc;
+    for (int b = 0; b < o.k; ++b) {
+      r->mpos[b] = rank_in(P, o.qubits[b]);
+      loc.push_back(r->mpos[b]);
+    }
+    std::sort(loc.begin(), loc.end());
+    for (int b = 0; b < o.k; ++b) r->pos[b] = loc[b];
+    const std::size_t bytes = (std::size_t{1} << (2 * o.k)) * 16;
+    r->mat_offset = (uint32_t)mats->size();  // relative; fixed up by the caller
+    mats->insert(mats->end(), plan.pool.begin() + o.pool_offset,
+                 plan.pool.begin() + o.pool_offset + bytes);
+    return true;
+  }
+  r->kind = o.kind == Kind::Phase       ? ketk::RUN_PHASE
+            : o.pair_op == ketk::OP_CH   ? ketk::RUN_CH
+            : o.pair_op == ketk::OP_SWAP ? ketk::RUN_SWAP
+                                         : ketk::RUN_DENSE;
+  for (int i = 0; i < 8; ++i) r->m[i] = o.m[i];
+  int np = 0;
+  for (int i = 0; i < o.spec.npos; ++i) {
+    const int p = o.spec.pos[i];
+    if (P & bit(p)) {
+      r->pos[np++] = rank_in(P, p);
+    } else {
+      if (!(o.spec.set_mask & bit(p))) return false;
+      r->out_mask |= bit(p);
+    }
+  }
+  r->npos = np;
+  r->set = compress(o.spec.set_mask & P, P);
+  r->partner = compress(o.spec.partner, P);
+  return true;
+}
+
+}  // namespace
+
+void group_runs(Plan& plan, int n, int precision) {
+  const int tb = ketk::run_tile_bits(precision);
+  if (n < tb) return;
+  // the lowest bits every tile keeps: HBM rows of 2^low amplitudes
+  const int tuned = ketk::get_tuning(precision).run_low_bits;
+  const int low = std::min(tuned ? tuned : (precision == 0 ? 3 : 4), tb - 1);
+  const uint64_t low_mask = bit(low) - 1;
+  std::vector<Op> in;
+  in.swap(plan.ops);
+  std::size_t i = 0;
+  while (i < in.size()) {
+    uint64_t moved = 0;
+    std::size_t j = i;
+    double f = 0.0;
+    for (; j < in.size(); ++j) {
+      const int64_t mb = moved_bits(in[j]);
+      if (mb < 0) break;
+      const uint64_t m2 = moved | (uint64_t)mb;
+      if (__builtin_popcountll(m2 & ~low_mask) > tb - low) break;
+      moved = m2;
+      f += in[j].line_fraction;
+    }
+    // A run moves the whole state once; its passes alone move f sweeps.
+    // Below 2^22 amplitudes the state sits in the caches and launches
+    // dominate, so any two passes are worth a run there.
+    const bool worth = j - i >= 2 && (f > 1.2 || n <= 22);
+    uint64_t P = 0;
+    std::vector<ketk::RunOp> ops;
+    std::vector<unsigned char> mats;
+    bool ok = worth;
+    if (ok) {
+      P = tile_positions(moved, tb);
+      for (std::size_t t = i; t < j && ok; ++t) {
+        ketk::RunOp r;
+        ok = local_op(plan, in[t], P, &r, &mats);
+        ops.push_back(r);
+      }
+    }
+    if (!ok) {
+      plan.ops.push_back(in[i]);
+      ++i;
+      continue;
+    }
+    const std::size_t head = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp);
+    for (auto& r : ops)
+      if (r.kind == ketk::RUN_MATRIX) r.mat_offset += (uint32_t)head;
+    const std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
+    plan.pool.resize(off + head + mats.size());
+    ketk::RunRecord h{};
+    h.pmask = P;
+    h.n_ops = (int)ops.size();
+    std::memcpy(plan.pool.data() + off, &h, sizeof h);
+    std::memcpy(plan.pool.data() + off + sizeof h, ops.data(), ops.size() * sizeof(ketk::RunOp));
+    if (!mats.empty()) std::memcpy(plan.pool.data() + off + head, mats.data(), mats.size());
+    Op o;
+    o.kind = Kind::Run;
+    o.pool_offset = off;
+    o.line_fraction = 1.0;
+    o.run_pmask = P;
+    o.run_ops = (int)ops.size();
+    const double dim = std::ldexp(1.0, n);
+    for (std::size_t t = i; t < j; ++t) {
+      const Op& x = in[t];
+      o.n_gates += x.n_gates;
+      double touched = dim;
+      if (x.kind == Kind::Phase) touched = (double)x.spec.n_items;
+      else if (x.kind == Kind::Pair) touched = 2.0 * (double)x.spec.n_items;
+      o.flops_per_amp += x.flops_per_amp * touched / dim;
+    }
+    plan.ops.push_back(o);
+    i = j;
+  }
+}
+
+namespace {
+
 struct Pending {
   const ket_gate* g;
 };
@@ -407,6 +571,11 @@ void op_features(const Op& o, int n, int precision, float* f) {
       width = o.spec.npos;
       f[2] = 1.f;
       break;
+    case Kind::Run:
+      qm = o.run_pmask;
+      width = std::min(o.run_ops, 64) / 8;
+      f[4] = 1.f;
+      break;
   }
   const double sweep = 2.0 * (precision == 0 ? 16.0 : 8.0) * (double)(uint64_t{1} << n);
   f[5] = width / 6.f;
@@ -424,7 +593,7 @@ void op_features(const Op& o, int n, int precision, float* f) {
 
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 const BlockDecider& decide, bool force_blocks) {
+                 const BlockDecider& decide, bool force_blocks, bool features_only) {
   const int L = line_bits(precision);
   const uint64_t all = n >= 64 ? ~uint64_t{0} : (bit(n) - 1);
   std::vector<std::size_t> rem(run);
@@ -500,6 +669,26 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
     std::vector<int> bq;
     for (int qb = 0; qb < 64; ++qb)
       if (Q & bit(qb)) bq.push_back(qb);
+    if (features_only) {
+      const cx id[4] = {{1.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}};
+      Op block;
+      if (k == 1 && trailing.empty() && !force_blocks) {
+        block = pair_op(n, L, ketk::OP_DENSE, bq[0], {}, id);
+      } else {
+        block.kind = Kind::Block;
+        block.k = k;
+        for (int b = 0; b < k; ++b) block.qubits[b] = bq[b];
+        // op_for_block folds the terms with every bit inside the block
+        for (const Term& t : terms)
+          if ((t.mask & ~Q) != 0 || (t.mask & Q) == 0) ++block.n_terms;
+        block.line_fraction = 1.0;
+        block.flops_per_amp = 8.0 * (double)(1 << k);
+      }
+      block.n_gates = (int)order.size();
+      if (decide) (void)decide(ch, block, singles);
+      plan->ops.push_back(block);
+      continue;
+    }
     std::vector<kethost::GateRef> refs;
     for (std::size_t gi : members) {
       const ket_gate& g = gates[gi];
@@ -563,7 +752,7 @@ void learned_choice(const ket_gate* gates, std::size_t n_gates, int n, int preci
     std::vector<std::size_t> run;
     for (std::size_t i = 0; i <= n_gates; ++i) {
       if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
-        if (!run.empty()) pack_blocks(gates, run, n, precision, k, &scratch, rec);
+        if (!run.empty()) pack_blocks(gates, run, n, precision, k, &scratch, rec, false, true);
         run.clear();
         continue;
       }
@@ -610,10 +799,11 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
     if (st != KET_OK) return st;
     if (kethost::is_unitary(gates[i].type)) ++plan.gates;
   }
-  if (fusion == KET_FUSE_NONE) {
+  if (fusion == KET_FUSE_NONE || fusion == KET_FUSE_EXACT) {
     for (std::size_t i = 0; i < n_gates; ++i)
       if (kethost::is_unitary(gates[i].type))
         plan.ops.push_back(op_for_gate(gates[i], n, precision));
+    if (fusion == KET_FUSE_EXACT) group_runs(plan, n, precision);
     return KET_OK;
   }
   if (fusion == KET_FUSE_REFERENCE) {
@@ -634,6 +824,7 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
       pend.push_back(&g);
     }
     fuser_flush(pend, bq, n, precision, &plan);
+    if (ketk::get_tuning(precision).exact_runs) group_runs(plan, n, precision);
     return KET_OK;
   }
   if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_LEARNED || fusion == KET_FUSE_BLOCKS_ALL) {

@@ -12,7 +12,7 @@
 
 namespace ketplan {
 
-enum class Kind { Pair, Phase, MatrixExact, Block };
+enum class Kind { Pair, Phase, MatrixExact, Block, Run };
 
 struct Op {
   Kind kind = Kind::Pair;
@@ -26,6 +26,8 @@ struct Op {
   double flops_per_amp = 0.0;
   int n_gates = 0;                          // unitary gates this pass covers
   int n_terms = 0;                          // Block: trailing phase terms
+  uint64_t run_pmask = 0;                   // Run: the tile positions P
+  int run_ops = 0;                          // Run: passes it applies
 };
 
 // A diagonal phase term: multiply by exp(i * angle) where all bits of mask
@@ -57,6 +59,13 @@ inline int default_block_k(int precision) {
   if (t) return t;
   return precision == 0 ? 4 : 5;
 }
+
+// Groups consecutive exact passes (Pair, Phase, MatrixExact) into runs
+// applied on LDS tiles (ketk::launch_run) where that moves fewer bytes:
+// each run's tile positions hold every bit its passes move plus the lowest
+// others, at least ket_tuning.run_low_bits of them low contiguous bits.
+// Results are unchanged bit for bit. No-op below run_tile_bits qubits.
+void group_runs(Plan& plan, int n, int precision);
 
 // Lowest qubit of a MatrixExact or Block op.
 inline int lowest_qubit(const Op& o) {
@@ -116,9 +125,12 @@ bool predict_log_ms(const std::vector<float>& feats, std::vector<float>* out);
 
 // The k-qubit packer (KET_FUSE_BLOCKS) over the unitary gate indices `run`.
 // force_blocks: every packed group becomes a Block pass (one kernel kind
-// for the whole plan, as the batched engine needs).
+// for the whole plan, as the batched engine needs). features_only: block
+// passes carry what op_features reads but no matrix or pool record (for
+// deciders that only record; the plan cannot be launched).
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
-                 const BlockDecider& decide, bool force_blocks = false);
+                 const BlockDecider& decide, bool force_blocks = false,
+                 bool features_only = false);
 
 }  // namespace ketplan

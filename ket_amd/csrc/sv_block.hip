@@ -977,6 +977,31 @@ __device__ __forceinline__ void run_op(ketd::cplx<T>* L, const RunOp& op,
   }
 }
 
+// A stretch of consecutive phase ops: diagonal, so each thread applies all
+// of them to the amplitudes it owns (tile indices tid + 256 j) in registers,
+// in op order, with one LDS read and write for the whole stretch and no
+// barrier in between.
+template <class T, int TB>
+__device__ __forceinline__ void run_phases(ketd::cplx<T>* L, const RunOp* ops, int count,
+                                           uint64_t fixed) {
+  constexpr int PER = (1 << TB) / kThreads;
+  ketd::cplx<T> v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) v[j] = L[threadIdx.x + j * kThreads];
+  for (int q = 0; q < count; ++q) {
+    const RunOp& op = ops[q];
+    if ((fixed & op.out_mask) != op.out_mask) continue;  // uniform
+    const ketd::cplx<T> ph = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t i = threadIdx.x + j * kThreads;
+      if ((i & op.set) == op.set) v[j] = ketd::cmul<T>(v[j], ph);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) L[threadIdx.x + j * kThreads] = v[j];
+}
+
 // One workgroup per tile (grid-stride): load the tile in index order, apply
 // the ops whose outside controls the tile satisfies, store it back.
 template <class T, int TB>
@@ -997,6 +1022,13 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
     io.load(s, fixed, L);
     for (int o = 0; o < h.n_ops; ++o) {
       __syncthreads();
+      if (ops[o].kind == RUN_PHASE) {
+        int e = o + 1;
+        while (e < h.n_ops && ops[e].kind == RUN_PHASE) ++e;
+        run_phases<T, TB>(L, ops + o, e - o, fixed);
+        o = e - 1;
+        continue;
+      }
       if ((fixed & ops[o].out_mask) != ops[o].out_mask) continue;  // uniform
       run_op<T, TB>(L, ops[o], rec);
     }

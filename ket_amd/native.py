@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libket_amd.so")
 
 C128, C64 = 0, 1
-FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED, FUSE_BLOCKS_ALL = 0, 1, 2, 3, 4
+FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED, FUSE_BLOCKS_ALL, FUSE_EXACT = 0, 1, 2, 3, 4, 5
 
 STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: MemoryError,
           5: NotImplementedError}
@@ -35,7 +35,7 @@ class LaunchRecord(ctypes.Structure):
                 ("ms", ctypes.c_double), ("features", ctypes.c_float * MLP_INPUTS)]
 
 
-LAUNCH_KINDS = ["dense1q", "ch", "permute", "phase", "matrix", "block"]
+LAUNCH_KINDS = ["dense1q", "ch", "permute", "phase", "matrix", "block", "run"]
 
 
 class BatchStats(ctypes.Structure):
@@ -146,11 +146,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_int32), ("nontemporal", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("block_tiles", ctypes.c_int32),
                 ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32),
-                ("pair_exchange", ctypes.c_int32), ("block_k", ctypes.c_int32)]
+                ("pair_exchange", ctypes.c_int32), ("block_k", ctypes.c_int32),
+                ("exact_runs", ctypes.c_int32), ("run_low_bits", ctypes.c_int32)]
 
 
 TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tiles", "block_bpc",
-                 "block_stage", "pair_exchange", "block_k")
+                 "block_stage", "pair_exchange", "block_k", "exact_runs", "run_low_bits")
 
 
 def get_tuning(precision: int = C128) -> dict:

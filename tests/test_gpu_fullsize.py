@@ -64,3 +64,15 @@ def test_exact_sampling_30q_known_support():
     assert np.array_equal(shots, want)
     counts = np.array([np.sum(shots == s) for s in support])
     assert counts.min() > 625 - 5 * 25 and counts.max() < 625 + 5 * 25
+
+
+def test_qft30_exact_runs_equal_per_gate_passes():
+    """QFT-30 as exact runs on LDS tiles equals one pass per gate bit for bit
+    over the whole 16 GiB state (max |difference| exactly 0)."""
+    c = W.qft_u(30, seed=12)
+    with N.StateVector(30, N.C128) as a, N.StateVector(30, N.C128) as b:
+        a.apply_circuit(c, N.FUSE_NONE)
+        st = b.apply_circuit(c, N.FUSE_EXACT)
+        assert st["launches"] < c.n_unitary() // 4
+        f, dmax = a.fidelity(b)
+        assert dmax == 0.0 and abs(f - 1.0) <= 1e-12

@@ -86,3 +86,5 @@ def test_learned_planner_beats_ket_fuser():
     print("learned vs ket's Fuser:", res)
     assert res["reference_fuser_circuits"] == len(circs)
     assert res["speedup_vs_reference_fuser"] >= 2.0
+    # ket's Fuser sped up by exact runs on LDS tiles is still slower
+    assert res["speedup_vs_reference_fuser_runs"] >= 1.2

@@ -92,7 +92,7 @@ CIRCUITS = {
 
 
 @pytest.mark.parametrize("cname", sorted(CIRCUITS))
-@pytest.mark.parametrize("fusion", [N.FUSE_NONE, N.FUSE_REFERENCE])
+@pytest.mark.parametrize("fusion", [N.FUSE_NONE, N.FUSE_REFERENCE, N.FUSE_EXACT])
 def test_circuit_exact(cname, fusion):
     c = CIRCUITS[cname]()
     want = O.run(c, fuse=(fusion == N.FUSE_REFERENCE))
@@ -318,3 +318,27 @@ def test_tuned_block_width_applies_when_call_passes_zero(precision, restore_tuni
             sv.set_basis(0)
             sv.apply_circuit(c, N.FUSE_BLOCKS, k)
             assert np.array_equal(auto, sv.to_numpy())
+
+
+@pytest.mark.parametrize("precision", [N.C128, N.C64])
+@pytest.mark.parametrize("cname", ["mixed12", "qft14", "bwm13", "diag12", "brick14"])
+def test_exact_runs_equal_per_gate_passes(cname, precision, restore_tuning):
+    """Consecutive passes applied as exact runs on LDS tiles (KET_FUSE_EXACT;
+    ket's Fuser with exact_runs on) give the bits of one pass per gate, from
+    a dense random state, in both precisions."""
+    c = CIRCUITS[cname]()
+    n = c.n_qubits
+    start = random_state(n, 40 + n)
+    with N.StateVector(n, precision) as sv:
+        out = {}
+        for label, fusion, runs in (("none", N.FUSE_NONE, 1), ("exact", N.FUSE_EXACT, 1),
+                                    ("ref_off", N.FUSE_REFERENCE, 0), ("ref_on", N.FUSE_REFERENCE, 1)):
+            N.set_tuning(precision, exact_runs=runs)
+            sv.load(start)
+            sv.apply_circuit(c, fusion)
+            out[label] = sv.to_numpy()
+        assert np.array_equal(out["exact"], out["none"])
+        assert np.array_equal(out["ref_on"], out["ref_off"])
+        if n >= (11 if precision == N.C128 else 12):
+            kinds = [r["kind"] for r in sv.profile_circuit(c, N.FUSE_EXACT)]
+            assert kinds.count("run") >= 1

@@ -1,0 +1,75 @@
+"""Exact runs on the CPU (tests/cpp/test_runs.cpp, no GPU): circuits planned
+one pass per gate and with consecutive passes grouped into runs on LDS tiles
+are both executed on the CPU, the runs through their records exactly as
+k_run_tile walks them, and must give equal bits; for KET_FUSE_NONE against
+KET_FUSE_EXACT and for ket's Fuser with runs off against on."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+from ket_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ket_amd", "test_runs")
+
+CIRCUITS = {
+    "qft14": lambda: W.qft_u(14, seed=1),
+    "brick13": lambda: W.brickwork(13, 6, seed=2, measure=False),
+    "mixed14": lambda: W.random_mixed(14, 10, seed=3),
+    "diag13": lambda: W.diag_mix(13, 8, seed=4),
+    "mixed12": lambda: W.random_mixed(12, 12, seed=5),
+    "bwm14": lambda: W.brickwork_mixed(14, 8, seed=6),
+    "qft12": lambda: W.qft_u(12, seed=7),
+}
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
+@pytest.mark.parametrize("mode", ["none", "reference"])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", sorted(CIRCUITS))
+def test_runs_equal_per_gate_passes(name, precision, mode):
+    c = CIRCUITS[name]()
+    if c.n_qubits < (11 if precision == 0 else 12):
+        pytest.skip("below the tile size: no runs")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c.bin")
+        c.records().tofile(path)
+        r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), mode],
+                           capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ok, passes, runs, in_runs = r.stdout.split()
+    assert ok == "ok" and int(runs) >= 1 and int(in_runs) >= 2
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
+@pytest.mark.parametrize("low", [3, 4, 7, 9])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_runs_equal_per_gate_passes_other_tile_shapes(precision, low):
+    """Runs planned with other numbers of low tile bits (ket_tuning.run_low_bits)."""
+    for name in ("qft14", "mixed14", "bwm14"):
+        c = CIRCUITS[name]()
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "c.bin")
+            c.records().tofile(path)
+            r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), "none", str(low)],
+                               capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0 and r.stdout.startswith("ok "), r.stdout + r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_learned_planner_recording_features_match_full_packing(precision):
+    """The learned planner records candidate passes without building block
+    matrices; their cost-model features must equal the full packer's."""
+    from ket_amd import costmodel as CM
+    circs = CM.config5_circuits(4, seed=99, lo=20, hi=24) + [W.qft_u(20, 1), W.diag_mix(18, 10, 2),
+                                                             W.hea(16, 8, 3)]
+    with tempfile.TemporaryDirectory() as d:
+        for c in circs:
+            path = os.path.join(d, "c.bin")
+            c.records().tofile(path)
+            r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), "features"],
+                               capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0 and r.stdout.strip() == "ok features", r.stdout + r.stderr
