@@ -297,9 +297,11 @@ typedef struct ket_tuning {
                                KET_FUSE_LEARNED calls (and batches) that pass
                                block_k = 0: 2..6, or 0 for the built-in 4
                                (complex128) / 5 (complex64) */
-  int32_t exact_runs;       /* KET_FUSE_REFERENCE groups consecutive passes
-                               into exact runs on LDS tiles: 1 (default) or
-                               0 (one launch per pass) */
+  int32_t exact_runs;       /* KET_FUSE_REFERENCE, and the per-gate passes
+                               KET_FUSE_BLOCKS / KET_FUSE_LEARNED leave
+                               between blocks, group consecutive passes into
+                               exact runs on LDS tiles: 1 (default) or 0 (one
+                               launch per pass) */
   int32_t run_low_bits;     /* exact runs: the lowest index bits every tile
                                keeps, so its HBM rows are 2^run_low_bits
                                amplitudes long (3..10; 0: the default) */

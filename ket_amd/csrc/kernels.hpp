@@ -37,7 +37,8 @@ struct Tuning {
                      // this: one amplitude per thread, lane-pair swap
   int block_k;       // fused block width when a call passes 0 (0: 4 for
                      // complex128, 5 for complex64)
-  int exact_runs;    // KET_FUSE_REFERENCE: group passes into exact runs
+  int exact_runs;    // group consecutive exact passes into runs (ket's Fuser,
+                     // and the passes the block packer leaves)
   int run_low_bits;  // exact runs: low index bits kept in every tile
 };
 // Per precision (0 = complex128, 1 = complex64).

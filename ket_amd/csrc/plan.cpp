@@ -847,6 +847,10 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
       }
       run.push_back(i);
     }
+    // The per-gate passes the packer leaves between blocks run as exact
+    // runs where they are consecutive (their values do not change).
+    if (fusion != KET_FUSE_BLOCKS_ALL && ketk::get_tuning(precision).exact_runs)
+      group_runs(plan, n, precision);
     return KET_OK;
   }
   *err = "ket_amd: unknown fusion mode " + std::to_string(fusion);

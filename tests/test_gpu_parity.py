@@ -324,21 +324,25 @@ def test_tuned_block_width_applies_when_call_passes_zero(precision, restore_tuni
 @pytest.mark.parametrize("cname", ["mixed12", "qft14", "bwm13", "diag12", "brick14"])
 def test_exact_runs_equal_per_gate_passes(cname, precision, restore_tuning):
     """Consecutive passes applied as exact runs on LDS tiles (KET_FUSE_EXACT;
-    ket's Fuser with exact_runs on) give the bits of one pass per gate, from
-    a dense random state, in both precisions."""
+    ket's Fuser and the block packer's leftover passes with exact_runs on)
+    give the bits of one pass per gate, from a dense random state, in both
+    precisions."""
     c = CIRCUITS[cname]()
     n = c.n_qubits
     start = random_state(n, 40 + n)
     with N.StateVector(n, precision) as sv:
         out = {}
         for label, fusion, runs in (("none", N.FUSE_NONE, 1), ("exact", N.FUSE_EXACT, 1),
-                                    ("ref_off", N.FUSE_REFERENCE, 0), ("ref_on", N.FUSE_REFERENCE, 1)):
+                                    ("ref_off", N.FUSE_REFERENCE, 0), ("ref_on", N.FUSE_REFERENCE, 1),
+                                    ("blocks_off", N.FUSE_BLOCKS, 0), ("blocks_on", N.FUSE_BLOCKS, 1)):
             N.set_tuning(precision, exact_runs=runs)
             sv.load(start)
             sv.apply_circuit(c, fusion)
             out[label] = sv.to_numpy()
         assert np.array_equal(out["exact"], out["none"])
         assert np.array_equal(out["ref_on"], out["ref_off"])
+        # the passes the block packer leaves, as runs
+        assert np.array_equal(out["blocks_on"], out["blocks_off"])
         if n >= (11 if precision == N.C128 else 12):
             kinds = [r["kind"] for r in sv.profile_circuit(c, N.FUSE_EXACT)]
             assert kinds.count("run") >= 1
