@@ -304,7 +304,8 @@ typedef struct ket_tuning {
                                launch per pass) */
   int32_t run_low_bits;     /* exact runs: the lowest index bits every tile
                                keeps, so its HBM rows are 2^run_low_bits
-                               amplitudes long (3..10; 0: the default) */
+                               amplitudes long (3..10; default 3 complex128,
+                               4 complex64) */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

@@ -909,7 +909,11 @@ __device__ __forceinline__ void run_matrix(ketd::cplx<T>* L, const RunOp& op,
       if ((e >> b) & 1) o |= 1u << op.mpos[b];
     off[e] = o;
   }
+#pragma unroll 1
   for (uint32_t g = threadIdx.x; g < (1u << (TB - K)); g += kThreads) {
+    // a compiler-only fence: keeps the matrix reads (scalar loads) inside
+    // the loop rather than hoisted into registers for the whole op
+    __asm__ volatile("" ::: "memory");
     const uint32_t base = (uint32_t)ketd::insert_zeros(g, K, op.pos);
     ketd::cplx<T> in[D];
 #pragma unroll

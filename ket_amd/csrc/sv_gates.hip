@@ -265,9 +265,10 @@ cplx<T> to_c(const double* m, int i) {
 // for partners fewer than 2^16 (complex128) or 2^4 (complex64) amplitudes
 // away.
 // Fused blocks: 32 workgroups per CU; complex64 blocks always through LDS
-// tiles, complex128 ones when their lowest qubit is 0..2 (and the launch rule
-// of launch_block allows), with nontemporal tile transfers.
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 64, 16, 0, 1, 3}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4}};
+// tiles, complex128 ones when their lowest qubit is 0..2 (always-tiles
+// measured within 0.1%), with nontemporal tile transfers. Exact runs keep
+// 3 (complex128) / 4 (complex64) low bits in every tile.
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4}};
 
 // True when every 128-byte line the pass writes is written whole: no
 // condition bit (control, phase bit, the bits of a SWAP) below the line's
