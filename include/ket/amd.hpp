@@ -3,9 +3,9 @@
 // Defaults reproduce ket::run value for value: complex128 amplitudes and
 // ket's own fusion rule. The environment can set them too:
 //   KET_AMD_PRECISION = c128 | c64
-//   KET_AMD_FUSION    = reference | none | blocks | learned
-//   KET_AMD_BLOCK_K   = 2..6 (width of fused blocks; default 4 for complex128,
-//                       5 for complex64)
+//   KET_AMD_FUSION    = reference | none | exact | blocks | learned
+//   KET_AMD_BLOCK_K   = 2..6 (width of fused blocks; default: the tuned
+//                       width, 4 for complex128, 5 for complex64)
 #pragma once
 
 #include <cstddef>
@@ -22,12 +22,16 @@ class Circuit;
 namespace ket::amd {
 
 enum class Precision { Complex128, Complex64 };
-enum class Fusion { None, Reference, Blocks, Learned };
+// None: one pass per gate. Reference: ket's Fuser rule (with complex128,
+// equal to ket::run value for value). Exact: None's arithmetic with
+// consecutive passes applied on LDS tiles (equal to None bit for bit).
+// Blocks, Learned: dense fused blocks (rounding differs).
+enum class Fusion { None, Reference, Blocks, Learned, Exact };
 
 struct Options {
   Precision precision = Precision::Complex128;
   Fusion fusion = Fusion::Reference;
-  int block_k = 0;  // 0: 4 for complex128, 5 for complex64
+  int block_k = 0;  // 0: the tuned width (4 for complex128, 5 for complex64)
 };
 
 void set_options(const Options& options);

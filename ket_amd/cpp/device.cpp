@@ -101,6 +101,7 @@ int fusion_code(amd::Fusion f) {
     case amd::Fusion::None: return KET_FUSE_NONE;
     case amd::Fusion::Blocks: return KET_FUSE_BLOCKS;
     case amd::Fusion::Learned: return KET_FUSE_LEARNED;
+    case amd::Fusion::Exact: return KET_FUSE_EXACT;
     default: return KET_FUSE_REFERENCE;
   }
 }
@@ -124,6 +125,7 @@ void init_from_env() {
     if (!std::strcmp(f, "none")) g_opt.fusion = amd::Fusion::None;
     else if (!std::strcmp(f, "blocks")) g_opt.fusion = amd::Fusion::Blocks;
     else if (!std::strcmp(f, "learned")) g_opt.fusion = amd::Fusion::Learned;
+    else if (!std::strcmp(f, "exact")) g_opt.fusion = amd::Fusion::Exact;
     else g_opt.fusion = amd::Fusion::Reference;
   }
   if (const char* k = std::getenv("KET_AMD_BLOCK_K")) {

@@ -71,6 +71,17 @@ int main() {
       }
       check(1.0 - std::norm(ov) / (na * nb) <= 1e-6, "complex64 option within 1-F <= 1e-6");
     }
+    // Exact: None's per-gate arithmetic on LDS tiles, equal bit for bit.
+    ket::amd::Options none, exact_runs;
+    none.fusion = ket::amd::Fusion::None;
+    exact_runs.fusion = ket::amd::Fusion::Exact;
+    ket::amd::set_options(none);
+    const ket::State per_gate = ket::run(c);
+    ket::amd::set_options(exact_runs);
+    const ket::State runs = ket::run(c);
+    bool equal = per_gate.size() == runs.size();
+    for (std::size_t i = 0; equal && i < runs.size(); ++i) equal = runs[i] == per_gate[i];
+    check(equal, "Fusion::Exact equals Fusion::None bit for bit");
     ket::amd::set_options(ket::amd::Options{});
     const ket::State again = ket::run(c);
     bool exact = true;

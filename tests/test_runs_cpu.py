@@ -22,6 +22,10 @@ CIRCUITS = {
     "mixed12": lambda: W.random_mixed(12, 12, seed=5),
     "bwm14": lambda: W.brickwork_mixed(14, 8, seed=6),
     "qft12": lambda: W.qft_u(12, seed=7),
+    # one tile holds the whole state (n = 11 complex128, 12 complex64)
+    "mixed11": lambda: W.random_mixed(11, 14, seed=8),
+    "clifford13": lambda: W.random_clifford(13, 60, seed=9),
+    "hea15": lambda: W.hea(15, 8, seed=10),
 }
 
 
