@@ -984,7 +984,8 @@ __device__ __forceinline__ void run_op(ketd::cplx<T>* L, const RunOp& op,
 // A stretch of consecutive phase ops: diagonal, so each thread applies all
 // of them to the amplitudes it owns (tile indices tid + 256 j) in registers,
 // in op order, with one LDS read and write for the whole stretch and no
-// barrier in between.
+// barrier in between. A condition on tile bits 8 and up holds or fails for
+// a whole register j (a scalar test), so such registers are skipped.
 template <class T, int TB>
 __device__ __forceinline__ void run_phases(ketd::cplx<T>* L, const RunOp* ops, int count,
                                            uint64_t fixed) {
@@ -996,11 +997,11 @@ __device__ __forceinline__ void run_phases(ketd::cplx<T>* L, const RunOp* ops, i
     const RunOp& op = ops[q];
     if ((fixed & op.out_mask) != op.out_mask) continue;  // uniform
     const ketd::cplx<T> ph = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
+    const uint32_t shi = op.set & ~(uint32_t)(kThreads - 1), slo = op.set & (kThreads - 1);
+    const bool lane = (threadIdx.x & slo) == slo;
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const uint32_t i = threadIdx.x + j * kThreads;
-      if ((i & op.set) == op.set) v[j] = ketd::cmul<T>(v[j], ph);
-    }
+    for (int j = 0; j < PER; ++j)
+      if ((((uint32_t)j * kThreads) & shi) == shi && lane) v[j] = ketd::cmul<T>(v[j], ph);
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j) L[threadIdx.x + j * kThreads] = v[j];
