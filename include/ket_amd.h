@@ -151,6 +151,12 @@ int ket_sv_renormalize(ket_sv* sv);
  * ket's measure() returns for that u: the first index whose sequential FP64
  * cumulative sum of |a|^2 exceeds u, else 2^n - 1. */
 int ket_sv_sample(ket_sv* sv, const double* u, uint64_t n, uint64_t* out);
+/* The probability prefix measure() scans (src/measurement.cpp:15-20): out[k]
+ * = the FP64 running sum cum after it adds |a_i|^2 for i = idx[k], summed in
+ * index order exactly as ket does, bit for bit. Builds the sampler's exact
+ * chunk starts when the state has changed since they were last built (the
+ * preparation ket_sv_sample also runs). idx[k] < 2^n (else KET_ERR_RANGE). */
+int ket_sv_prob_prefix(ket_sv* sv, const uint64_t* idx, uint64_t n, double* out);
 /* <psi|P|psi> for the masks ket's expval derives from a Pauli string:
  * flip = X|Y positions, sign = Z|Y positions, n_y = number of Y. */
 int ket_sv_expval_pauli(ket_sv* sv, uint64_t flip, uint64_t sign, int n_y,

@@ -190,6 +190,21 @@ void fill_stats(const ket_sv* sv, const ketplan::Plan& plan, double plan_s,
   if (st) *st = r;
 }
 
+// The sampler's exact chunk starts for the current state (built once per
+// state change).
+int prepare_sampler(ket_sv* sv) {
+  if (sv->sample_valid) return KET_OK;
+  if (!sv->sbuf) {
+    HIP_TRY(hipMalloc(&sv->sbuf, ketk::sample_buffer_bytes(sv->dim)), "hipMalloc(sampler)");
+  }
+  ketk::SampleBuffers b;
+  ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
+  HIP_TRY(ketk::launch_sample_prepare(sv->stream, sv->precision, sv->data, sv->dim, b),
+          "sample prepare");
+  sv->sample_valid = true;
+  return KET_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -594,20 +609,36 @@ int ket_sv_renormalize(ket_sv* sv) {
   return KET_OK;
 }
 
+int ket_sv_prob_prefix(ket_sv* sv, const uint64_t* idx, uint64_t n, double* out) {
+  if (int st = check_sv(sv)) return st;
+  if (n == 0) return KET_OK;
+  if (!idx || !out) return fail(KET_ERR_ARG, "ket_amd: null prefix buffers");
+  for (uint64_t k = 0; k < n; ++k)
+    if (idx[k] >= sv->dim) return fail(KET_ERR_RANGE, "ket_amd: prefix index out of range");
+  if (int st = prepare_sampler(sv)) return st;
+  ketk::SampleBuffers b;
+  ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
+  const uint64_t step = uint64_t{1} << 22;
+  int st = ensure(&sv->io, &sv->io_cap, (size_t)std::min(n, step) * 16, "hipMalloc(io)");
+  if (st) return st;
+  for (uint64_t o = 0; o < n; o += step) {
+    const uint64_t c = std::min(step, n - o);
+    uint64_t* id = static_cast<uint64_t*>(sv->io);
+    double* od = reinterpret_cast<double*>(id + c);
+    HIP_TRY(hipMemcpyAsync(id, idx + o, c * 8, hipMemcpyHostToDevice, sv->stream), "copy idx");
+    HIP_TRY(ketk::launch_prob_prefix(sv->stream, sv->precision, sv->data, b, id, c, od),
+            "prob prefix");
+    HIP_TRY(hipMemcpyAsync(out + o, od, c * 8, hipMemcpyDeviceToHost, sv->stream), "copy out");
+    HIP_TRY(hipStreamSynchronize(sv->stream), "sync");
+  }
+  return KET_OK;
+}
+
 int ket_sv_sample(ket_sv* sv, const double* u, uint64_t n, uint64_t* out) {
   if (int st = check_sv(sv)) return st;
   if (n == 0) return KET_OK;
   if (!u || !out) return fail(KET_ERR_ARG, "ket_amd: null sample buffers");
-  if (!sv->sample_valid) {
-    if (!sv->sbuf) {
-      HIP_TRY(hipMalloc(&sv->sbuf, ketk::sample_buffer_bytes(sv->dim)), "hipMalloc(sampler)");
-    }
-    ketk::SampleBuffers b;
-    ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
-    HIP_TRY(ketk::launch_sample_prepare(sv->stream, sv->precision, sv->data, sv->dim, b),
-            "sample prepare");
-    sv->sample_valid = true;
-  }
+  if (int st = prepare_sampler(sv)) return st;
   ketk::SampleBuffers b;
   ketk::sample_buffers_carve(sv->sbuf, sv->dim, &b);
   const uint64_t step = uint64_t{1} << 22;

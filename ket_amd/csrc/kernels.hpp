@@ -187,6 +187,11 @@ size_t sample_buffer_bytes(uint64_t dim);
 void sample_buffers_carve(void* base, uint64_t dim, SampleBuffers* b);
 hipError_t launch_sample_prepare(hipStream_t st, int precision, const void* s,
                                  uint64_t dim, const SampleBuffers& b);
+// out[k] = exact sequential prefix of |a|^2 through index idx[k] (chunk start
+// from the prepared buffers, then the chunk's terms in order).
+hipError_t launch_prob_prefix(hipStream_t st, int precision, const void* s,
+                              const SampleBuffers& b, const uint64_t* idx_dev, uint64_t n,
+                              double* out_dev);
 hipError_t launch_sample_draw(hipStream_t st, int precision, const void* s,
                               uint64_t dim, const SampleBuffers& b,
                               const double* u_dev, uint64_t n,

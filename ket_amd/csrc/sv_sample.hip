@@ -251,6 +251,22 @@ __global__ __launch_bounds__(256) void k_exact_prefix(
 }
 
 // One wave per draw.
+// ket's running sum at one index per thread: the exact chunk start, then
+// the chunk's probabilities up to and including idx in index order, the
+// additions measure() makes (src/measurement.cpp:15-20).
+template <class T>
+__global__ __launch_bounds__(256) void k_prob_prefix(const ketd::cplx<T>* s, int chunk_log,
+                                                      const double* exact, const uint64_t* idx,
+                                                      uint64_t n, double* out) {
+  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const uint64_t i = idx[q];
+  const uint64_t j = i >> chunk_log;
+  double c = exact[j];
+  for (uint64_t t = j << chunk_log; t <= i; ++t) c += prob<T>(s, t);
+  out[q] = c;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_draw(
     const ketd::cplx<T>* s, uint64_t dim, int chunk_log, uint64_t n_chunks,
@@ -372,6 +388,20 @@ hipError_t launch_sample_prepare(hipStream_t st, int precision, const void* s,
     k_chunk_binade<float><<<nc, 256, 0, st>>>(p, b.chunk_log, b.approx, b.e0, b.flags, b.msum);
     k_exact_prefix<float><<<1, 256, 0, st>>>(p, b.chunk_log, b.n_chunks, b.approx, b.e0, b.flags, b.msum, b.exact, b.mode);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_prob_prefix(hipStream_t st, int precision, const void* s,
+                              const SampleBuffers& b, const uint64_t* idx_dev, uint64_t n,
+                              double* out_dev) {
+  if (n == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (precision == 0)
+    k_prob_prefix<double><<<grid, 256, 0, st>>>(static_cast<const double2*>(s), b.chunk_log,
+                                                b.exact, idx_dev, n, out_dev);
+  else
+    k_prob_prefix<float><<<grid, 256, 0, st>>>(static_cast<const float2*>(s), b.chunk_log,
+                                               b.exact, idx_dev, n, out_dev);
   return hipGetLastError();
 }
 

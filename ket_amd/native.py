@@ -94,6 +94,7 @@ def lib():
             "ket_sv_norm": ([vp, P(dbl)], i32),
             "ket_sv_renormalize": ([vp], i32),
             "ket_sv_sample": ([vp, vp, u64, vp], i32),
+            "ket_sv_prob_prefix": ([vp, vp, u64, vp], i32),
             "ket_sv_expval_pauli": ([vp, u64, u64, i32, P(dbl)], i32),
             "ket_sv_expval_pauli_sum": ([vp, sz, vp, vp, vp, vp, P(dbl)], i32),
             "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
@@ -332,6 +333,13 @@ class StateVector:
         uu = np.ascontiguousarray(u, dtype=np.float64)
         out = np.zeros(uu.size, dtype=np.uint64)
         check(lib().ket_sv_sample(self._h, _p(uu), uu.size, _p(out)))
+        return out
+
+    def prob_prefix(self, idx) -> np.ndarray:
+        """ket's sequential running sum of |a|^2 through each index."""
+        ii = np.ascontiguousarray(idx, dtype=np.uint64)
+        out = np.zeros(ii.size, dtype=np.float64)
+        check(lib().ket_sv_prob_prefix(self._h, _p(ii), ii.size, _p(out)))
         return out
 
     def expval(self, pauli: str) -> float:

@@ -346,3 +346,35 @@ def test_exact_runs_equal_per_gate_passes(cname, precision, restore_tuning):
         if n >= (11 if precision == N.C128 else 12):
             kinds = [r["kind"] for r in sv.profile_circuit(c, N.FUSE_EXACT)]
             assert kinds.count("run") >= 1
+
+
+@pytest.mark.parametrize("precision", [N.C128, N.C64])
+@pytest.mark.parametrize("n", [3, 12, 15])
+def test_prob_prefix_is_kets_sequential_sum(n, precision):
+    """ket_sv_prob_prefix equals the FP64 running sum measure() builds
+    (numpy's add.accumulate is the same sequential sum) bit for bit, at
+    random indices, chunk boundaries and the last index; sampling agrees
+    with it (the outcome for u is the first index whose prefix exceeds u)."""
+    c = W.random_mixed(n, 6, seed=n)
+    with N.StateVector(n, precision) as sv:
+        sv.apply_circuit(c, N.FUSE_NONE)
+        a = sv.to_numpy()
+        if precision == N.C64:  # the state as stored, widened exactly
+            a = a.astype(np.complex64).astype(np.complex128)
+        p = a.real * a.real + a.imag * a.imag
+        cum = np.add.accumulate(p)
+        rng = np.random.default_rng(n)
+        dim = 1 << n
+        idx = np.unique(np.concatenate([rng.integers(0, dim, 300), [0, dim - 1],
+                                        np.arange(0, dim, 4096), np.arange(4095, dim, 4096)]))
+        got = sv.prob_prefix(idx)
+        assert np.array_equal(got, cum[idx])
+        u = O.uniforms(n, 200)
+        shots = sv.sample(u)
+        pre = sv.prob_prefix(shots)
+        inside = u < cum[-1]  # u past the total: ket returns the last index
+        assert np.all(u[inside] < pre[inside]) and np.all(shots[~inside] == dim - 1)
+        prev = sv.prob_prefix(np.maximum(shots.astype(np.int64) - 1, 0).astype(np.uint64))
+        assert np.all((shots == 0) | (u >= prev))
+        with pytest.raises(IndexError):
+            sv.prob_prefix([dim])
