@@ -66,3 +66,18 @@ def test_tuning_round_trip_and_validation():
     finally:
         for p, t in saved.items():
             N.set_tuning(p, **t)
+
+
+def test_default_tuning_is_the_recorded_autotuner_pick():
+    """The library's built-in launch shapes are the picks the autotuner
+    recorded on MI355X (ket_amd/tuning.json), field by field."""
+    import json
+    table = json.load(open(os.path.join(ROOT, "ket_amd", "tuning.json")))
+    for p, key, width in ((N.C128, "c128", 4), (N.C64, "c64", 5)):
+        got = N.get_tuning(p)
+        for section in ("", "_blocks", "_width", "_runs"):
+            for field, value in table[key + section]["best"].items():
+                if field == "block_k":  # 0 selects the built-in width
+                    assert value == width and got["block_k"] in (0, width), (key, field)
+                else:
+                    assert got[field] == value, (key, section, field, got[field], value)
