@@ -1,3 +1,5 @@
+// SPDX-License-Identifier: MIT
+// SPDX-FileCopyrightText: 2026 Breno Cunha Queiroz
 // ket public API (drop-in re-declaration; reference include/ket/circuit.hpp).
 #pragma once
 

@@ -1,3 +1,5 @@
+// SPDX-License-Identifier: MIT
+// SPDX-FileCopyrightText: 2026 Breno Cunha Queiroz
 // Umbrella header of the ket API served by the MI355X backend.
 #pragma once
 

@@ -1,14 +1,22 @@
 // OpenQASM 2.0 front end for the ket:: API (reference interface
 // include/ket/qasm.hpp: from_qasm / to_qasm; behaviour of
-// bindings/qasm/qasm.cpp:308-544 and :761): registers, the qelib1 gates ket
-// has, u/u1/u2/u3/p, measure (single and register-wide), barrier, user
-// `gate` definitions (appended as composite blocks), `pi` expressions with
-// + - * / ^, unary signs, parentheses and sin/cos/tan/exp/ln/sqrt. Anything
-// else (opaque, reset, if, classical control) throws std::runtime_error.
+// bindings/qasm/qasm.cpp:308-544 for parsing and :551-782 for writing).
+//
+// from_qasm: registers, the qelib1 gates ket has, U/u/u1/u2/u3/p/phase,
+// measure (single and register-wide), barrier, user `gate` definitions
+// (appended as composite blocks, usable before they are defined), `pi`
+// expressions with + - * / ^, unary signs, parentheses and
+// sin/cos/tan/exp/ln/sqrt. Anything else (opaque, reset, if, classical
+// control) throws std::runtime_error. Every input ket's parser accepts gives
+// the same circuit here.
+//
+// to_qasm: the text ket writes, byte for byte: one `gate` definition per
+// distinct composite (dependencies first), pi-fraction angles, probes left
+// out.
 //
 // Written as a tokenizer plus recursive-descent parser; the circuit is built
-// after a first pass has collected every register, so several qregs become
-// one qubit range in declaration order.
+// after a first pass has collected every register and a second every gate
+// definition, so several qregs become one qubit range in declaration order.
 #include <ket/qasm.hpp>
 
 #include <cctype>
@@ -16,6 +24,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -167,6 +176,18 @@ class Parser {
         }
       }
     }
+    // Pass 2: gate definitions, so a gate may be used before (or inside a
+    // definition written before) its own definition, as ket allows. The
+    // first definition of a name is kept (bindings/qasm/qasm.cpp:488-500).
+    while (peek().kind != Tok::End) {
+      if (peek().kind == Tok::Id && peek().text == "gate") {
+        next();
+        gate_def(true);
+      } else {
+        while (peek().kind != Tok::End && !accept(";")) next();
+      }
+    }
+    p_ = 0;
     Circuit c(nq);
     while (peek().kind != Tok::End) statement(c);
     return c;
@@ -302,7 +323,7 @@ class Parser {
       expect("]");
       expect(";");
     } else if (w == "gate") {
-      gate_def();
+      gate_def(false);
     } else if (w == "measure") {
       const auto q = operand(qregs_, line);
       expect("->");
@@ -324,8 +345,7 @@ class Parser {
     }
   }
 
-  void gate_def() {
-    const int line = peek().line;
+  void gate_def(bool keep) {
     const std::string name = ident();
     GateDef d;
     if (accept("(")) {
@@ -342,10 +362,6 @@ class Parser {
       GateDef::Stmt s;
       s.line = peek().line;
       s.name = ident();
-      if (s.name == "barrier") {
-        while (!accept(";")) next();
-        continue;
-      }
       if (accept("(")) {
         if (!accept(")")) {
           do s.params.push_back(expr());
@@ -358,8 +374,7 @@ class Parser {
       expect(";");
       d.body.push_back(std::move(s));
     }
-    if (defs_.count(name)) bad("gate '" + name + "' defined twice", line);
-    defs_[name] = std::move(d);
+    if (keep) defs_.emplace(name, std::move(d));
   }
 
   void apply(Circuit& c, const std::string& name, int line) {
@@ -392,40 +407,42 @@ class Parser {
 
   void emit(Circuit& c, const std::string& name, const std::vector<double>& p,
             const std::vector<std::size_t>& q, int line) {
+    // Gates without parameters ignore a parameter list, as ket's parser does.
+    constexpr std::size_t kAny = static_cast<std::size_t>(-1);
     auto need = [&](std::size_t nq, std::size_t np) {
-      if (q.size() != nq || p.size() != np)
+      if (q.size() != nq || (np != kAny && p.size() != np))
         bad("'" + name + "' takes " + std::to_string(nq) + " qubits and " +
                 std::to_string(np) + " parameters",
             line);
     };
     const double pi = 3.14159265358979323846;
-    if (name == "id") { need(1, 0); return; }
-    if (name == "h") { need(1, 0); c.h(q[0]); return; }
-    if (name == "x") { need(1, 0); c.x(q[0]); return; }
-    if (name == "y") { need(1, 0); c.y(q[0]); return; }
-    if (name == "z") { need(1, 0); c.z(q[0]); return; }
-    if (name == "s") { need(1, 0); c.s(q[0]); return; }
-    if (name == "sdg") { need(1, 0); c.sdg(q[0]); return; }
-    if (name == "t") { need(1, 0); c.t(q[0]); return; }
-    if (name == "tdg") { need(1, 0); c.tdg(q[0]); return; }
+    if (name == "id") { need(1, kAny); return; }
+    if (name == "h") { need(1, kAny); c.h(q[0]); return; }
+    if (name == "x") { need(1, kAny); c.x(q[0]); return; }
+    if (name == "y") { need(1, kAny); c.y(q[0]); return; }
+    if (name == "z") { need(1, kAny); c.z(q[0]); return; }
+    if (name == "s") { need(1, kAny); c.s(q[0]); return; }
+    if (name == "sdg") { need(1, kAny); c.sdg(q[0]); return; }
+    if (name == "t") { need(1, kAny); c.t(q[0]); return; }
+    if (name == "tdg") { need(1, kAny); c.tdg(q[0]); return; }
     if (name == "rx") { need(1, 1); c.rx(q[0], p[0]); return; }
     if (name == "ry") { need(1, 1); c.ry(q[0], p[0]); return; }
     if (name == "rz") { need(1, 1); c.rz(q[0], p[0]); return; }
     if (name == "u" || name == "U" || name == "u3") { need(1, 3); c.u(q[0], p[0], p[1], p[2]); return; }
     if (name == "u2") { need(1, 2); c.u(q[0], pi / 2, p[0], p[1]); return; }
-    if (name == "u1" || name == "p") { need(1, 1); c.u(q[0], 0.0, 0.0, p[0]); return; }
-    if (name == "cx" || name == "CX") { need(2, 0); c.cx(q[0], q[1]); return; }
-    if (name == "cy") { need(2, 0); c.cy(q[0], q[1]); return; }
-    if (name == "cz") { need(2, 0); c.cz(q[0], q[1]); return; }
-    if (name == "ch") { need(2, 0); c.ch(q[0], q[1]); return; }
+    if (name == "u1" || name == "p" || name == "phase") { need(1, 1); c.u(q[0], 0.0, 0.0, p[0]); return; }
+    if (name == "cx" || name == "CX") { need(2, kAny); c.cx(q[0], q[1]); return; }
+    if (name == "cy") { need(2, kAny); c.cy(q[0], q[1]); return; }
+    if (name == "cz") { need(2, kAny); c.cz(q[0], q[1]); return; }
+    if (name == "ch") { need(2, kAny); c.ch(q[0], q[1]); return; }
     if (name == "crx") { need(2, 1); c.crx(q[0], q[1], p[0]); return; }
     if (name == "cry") { need(2, 1); c.cry(q[0], q[1], p[0]); return; }
     if (name == "crz") { need(2, 1); c.crz(q[0], q[1], p[0]); return; }
     if (name == "cu1" || name == "cp") { need(2, 1); c.cp(q[0], q[1], p[0]); return; }
     if (name == "cu3" || name == "cu") { need(2, 3); c.cu(q[0], q[1], p[0], p[1], p[2]); return; }
-    if (name == "swap") { need(2, 0); c.swap(q[0], q[1]); return; }
-    if (name == "ccx") { need(3, 0); c.ccx(q[0], q[1], q[2]); return; }
-    if (name == "cswap") { need(3, 0); c.cswap(q[0], q[1], q[2]); return; }
+    if (name == "swap") { need(2, kAny); c.swap(q[0], q[1]); return; }
+    if (name == "ccx") { need(3, kAny); c.ccx(q[0], q[1], q[2]); return; }
+    if (name == "cswap") { need(3, kAny); c.cswap(q[0], q[1], q[2]); return; }
     auto it = defs_.find(name);
     if (it == defs_.end()) bad("unknown gate '" + name + "'", line);
     const GateDef& d = it->second;
@@ -450,7 +467,10 @@ class Parser {
         if (f == arg.end()) bad("unknown argument '" + a + "' in gate '" + name + "'", s.line);
         qs.push_back(f->second);
       }
-      emit(sub, s.name, vals, qs, s.line);
+      if (s.name == "barrier")
+        sub.barrier(qs);
+      else
+        emit(sub, s.name, vals, qs, s.line);
     }
     (void)line;
     return sub;
@@ -462,63 +482,170 @@ class Parser {
   std::map<std::string, GateDef> defs_;
 };
 
-void write_gates(std::ostringstream& os, const Circuit& c, const std::vector<std::size_t>& wire) {
-  static const char* names[] = {"h", "x", "y", "z", "s", "sdg", "t", "tdg", "rx", "ry", "rz",
-                                "u3", "ch", "cx", "cy", "cz", "crx", "cry", "crz", "cu3", "cu1",
-                                "swap", "ccx", "cswap"};
-  char buf[64];
-  for (const DagNode& node : c.dag().nodes()) {
-    const Gate& g = node.gate;
-    switch (g.type) {
-      case GateType::Measure:
-        os << "measure q[" << wire[g.qubits[0].index] << "] -> c[" << g.clbit << "];\n";
-        break;
-      case GateType::Barrier: {
-        os << "barrier ";
-        for (std::size_t i = 0; i < g.qubits.size(); ++i)
-          os << (i ? "," : "") << "q[" << wire[g.qubits[i].index] << "]";
-        os << ";\n";
-        break;
-      }
-      case GateType::Probe:
-        break;
-      case GateType::Composite: {
-        std::vector<std::size_t> sub(g.definition->n_qubits());
-        for (std::size_t j = 0; j < g.qubits.size(); ++j) sub[j] = wire[g.qubits[j].index];
-        write_gates(os, *g.definition, sub);
-        break;
-      }
-      default: {
-        os << names[static_cast<int>(g.type)];
-        if (!g.params.empty()) {
-          os << "(";
-          for (std::size_t i = 0; i < g.params.size(); ++i) {
-            std::snprintf(buf, sizeof buf, "%.17g", g.params[i]);
-            os << (i ? "," : "") << buf;
-          }
-          os << ")";
-        }
-        for (std::size_t i = 0; i < g.qubits.size(); ++i)
-          os << (i ? "," : " ") << "q[" << wire[g.qubits[i].index] << "]";
-        os << ";\n";
-      }
-    }
+// Angle text of the serializer. Multiples n*pi/d with d <= 12 (to 1e-9 of
+// the ratio) print as pi fractions, |angle| < 1e-12 as 0, anything else with
+// 15 significant digits. The rule fixes the text ket writes
+// (bindings/qasm/qasm.cpp:554-575), so it is kept as ket has it; the text
+// re-parses to the same text (pi fractions evaluate back within the 1e-9
+// window, 15-digit decimals print unchanged).
+std::string angle_text(double a) {
+  if (std::fabs(a) < 1e-12) return "0";
+  const double pi = std::acos(-1.0);
+  const double ratio = a / pi;
+  for (int d = 1; d <= 12; ++d) {
+    const double scaled = ratio * d;
+    const long n = std::lround(scaled);
+    if (n == 0 || !(std::fabs(scaled - static_cast<double>(n)) < 1e-9)) continue;
+    const long m = n < 0 ? -n : n;
+    std::string t = m == 1 ? std::string("pi") : std::to_string(m) + "*pi";
+    if (d > 1) t += "/" + std::to_string(d);
+    return n < 0 ? "-" + t : t;
+  }
+  char buf[40];
+  std::snprintf(buf, sizeof buf, "%.15g", a);
+  return buf;
+}
+
+// OpenQASM spelling of each GateType, and whether its parameters print.
+// ket writes U for the general one-qubit gate, cu3 for CU and cp for CP
+// (bindings/qasm/qasm.cpp:629-700).
+const char* gate_word(GateType t) {
+  switch (t) {
+    case GateType::H: return "h";
+    case GateType::X: return "x";
+    case GateType::Y: return "y";
+    case GateType::Z: return "z";
+    case GateType::S: return "s";
+    case GateType::Sdg: return "sdg";
+    case GateType::T: return "t";
+    case GateType::Tdg: return "tdg";
+    case GateType::Rx: return "rx";
+    case GateType::Ry: return "ry";
+    case GateType::Rz: return "rz";
+    case GateType::U: return "U";
+    case GateType::CH: return "ch";
+    case GateType::CX: return "cx";
+    case GateType::CY: return "cy";
+    case GateType::CZ: return "cz";
+    case GateType::CRx: return "crx";
+    case GateType::CRy: return "cry";
+    case GateType::CRz: return "crz";
+    case GateType::CU: return "cu3";
+    case GateType::CP: return "cp";
+    case GateType::Swap: return "swap";
+    case GateType::CCX: return "ccx";
+    case GateType::CSwap: return "cswap";
+    default: return nullptr;
   }
 }
+
+// Writes a circuit as OpenQASM. Every distinct composite becomes one `gate`
+// definition over formal qubits q0, q1, ... and is invoked by name; a
+// definition is written before any definition or statement that uses it
+// (rendering a body registers the composites inside it first). Two
+// composites share a definition when label, arity and rendered body agree;
+// a label reused with another body gets the first free suffix _2, _3, ...
+// Unlabelled composites are called `block`. Probes have no OpenQASM form.
+class Writer {
+ public:
+  std::string circuit(const Circuit& c) {
+    std::string body;
+    for (const DagNode& node : c.dag().nodes())
+      statement(body, node.gate, [](std::size_t q) { return "q[" + std::to_string(q) + "]"; });
+    std::string out = "OPENQASM 2.0;\ninclude \"qelib1.inc\";\n";
+    out += defs_;
+    out += "qreg q[" + std::to_string(c.n_qubits()) + "];\n";
+    if (c.n_clbits() > 0) out += "creg c[" + std::to_string(c.n_clbits()) + "];\n";
+    return out + body;
+  }
+
+ private:
+  template <class Ref>
+  void operands(std::string& os, const Gate& g, Ref ref) {
+    for (std::size_t k = 0; k < g.qubits.size(); ++k) {
+      os += k == 0 ? " " : ",";
+      os += ref(g.qubits[k].index);
+    }
+  }
+
+  template <class Ref>
+  void statement(std::string& os, const Gate& g, Ref ref) {
+    switch (g.type) {
+      case GateType::Probe:
+        return;
+      case GateType::Measure:
+        os += "measure " + ref(g.qubits[0].index) + " -> c[" + std::to_string(g.clbit) + "];\n";
+        return;
+      case GateType::Barrier:
+        os += "barrier";
+        if (g.qubits.empty()) os += " ";
+        operands(os, g, ref);
+        os += ";\n";
+        return;
+      case GateType::Composite:
+        if (!g.definition) return;
+        os += definition(*g.definition, g.label);
+        operands(os, g, ref);
+        os += ";\n";
+        return;
+      default:
+        break;
+    }
+    os += gate_word(g.type);
+    if (!g.params.empty()) {
+      os += "(";
+      for (std::size_t i = 0; i < g.params.size(); ++i) {
+        if (i) os += ",";
+        os += angle_text(g.params[i]);
+      }
+      os += ")";
+    }
+    operands(os, g, ref);
+    os += ";\n";
+  }
+
+  // The name a composite is invoked by; writes its definition on first use.
+  std::string definition(const Circuit& def, const std::string& label) {
+    std::string body;
+    for (const DagNode& node : def.dag().nodes())
+      statement(body, node.gate, [](std::size_t q) { return "q" + std::to_string(q); });
+    std::string key = label;
+    key += '#';
+    key += std::to_string(def.n_qubits());
+    key += '\x1f';
+    key += body;
+    if (auto it = named_.find(key); it != named_.end()) return it->second;
+    std::string name = label.empty() ? "block" : label;
+    if (taken_.count(name)) {
+      int k = 2;
+      while (taken_.count(name + "_" + std::to_string(k))) ++k;
+      name += "_" + std::to_string(k);
+    }
+    taken_.insert(name);
+    named_.emplace(std::move(key), name);
+    defs_ += "gate " + name;
+    for (std::size_t i = 0; i < def.n_qubits(); ++i) defs_ += (i ? ",q" : " q") + std::to_string(i);
+    defs_ += " {\n";
+    std::size_t a = 0;
+    while (a < body.size()) {
+      std::size_t b = body.find('\n', a);
+      if (b == std::string::npos) b = body.size();
+      if (b > a) defs_ += "  " + body.substr(a, b - a) + "\n";
+      a = b + 1;
+    }
+    defs_ += "}\n";
+    return name;
+  }
+
+  std::string defs_;
+  std::map<std::string, std::string> named_;
+  std::set<std::string> taken_;
+};
 
 }  // namespace
 
 Circuit from_qasm(const std::string& source) { return Parser(lex(source)).parse(); }
 
-std::string to_qasm(const Circuit& circuit) {
-  std::ostringstream os;
-  os << "OPENQASM 2.0;\ninclude \"qelib1.inc\";\n";
-  os << "qreg q[" << circuit.n_qubits() << "];\n";
-  if (circuit.n_clbits()) os << "creg c[" << circuit.n_clbits() << "];\n";
-  std::vector<std::size_t> wire(circuit.n_qubits());
-  for (std::size_t i = 0; i < wire.size(); ++i) wire[i] = i;
-  write_gates(os, circuit, wire);
-  return os.str();
-}
+std::string to_qasm(const Circuit& circuit) { return Writer().circuit(circuit); }
 
 }  // namespace ket
