@@ -166,6 +166,21 @@ int ket_sv_expval_pauli(ket_sv* sv, uint64_t flip, uint64_t sign, int n_y,
 int ket_sv_expval_pauli_sum(ket_sv* sv, size_t n_terms, const uint64_t* flip,
                             const uint64_t* sign, const int* n_y,
                             const double* coeff, double* out);
+/* ket_sv_expval_pauli_sum over 2^n_qubits amplitudes the caller holds in
+ * device memory (`amplitudes`, the layout above, KET_C128 or KET_C64), run on
+ * `stream` (a hipStream_t; NULL: the default stream) and waited for; same
+ * value bit for bit. It allocates nothing: no counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its
+ * reduction scratch is one fixed 32 KiB device array per GPU, held until the
+ * process ends, shared by every call on that GPU: calls that overlap on one
+ * GPU race. At most KET_NOALLOC_MAX_TERMS terms per call (more:
+ * KET_ERR_ARG). */
+#define KET_NOALLOC_MAX_TERMS 64
+int ket_expval_pauli_sum_noalloc(const void* amplitudes, int n_qubits, int precision,
+                                 size_t n_terms, const uint64_t* flip, const uint64_t* sign,
+                                 const int* n_y, const double* coeff, void* stream,
+                                 double* out);
 /* F = |<a|b>|^2 / (<a|a><b|b>) in FP64 and max_i |a_i - b_i| over two states
  * of one size (any precisions). */
 int ket_sv_fidelity(ket_sv* a, ket_sv* b, double* fidelity,

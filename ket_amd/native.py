@@ -98,6 +98,7 @@ def lib():
             "ket_sv_expval_pauli": ([vp, u64, u64, i32, P(dbl)], i32),
             "ket_sv_expval_pauli_sum": ([vp, sz, vp, vp, vp, vp, P(dbl)], i32),
             "ket_sv_fidelity": ([vp, vp, P(dbl), P(dbl)], i32),
+            "ket_expval_pauli_sum_noalloc": ([vp, i32, i32, sz, vp, vp, vp, vp, vp, P(dbl)], i32),
             "ket_sv_profile_circuit": ([vp, vp, sz, i32, i32, vp, sz, P(sz)], i32),
             "ket_set_tuning": ([vp], i32),
             "ket_set_tuning_for": ([i32, vp], i32),
