@@ -15,8 +15,15 @@ Besides the timed steps the run reports:
   * `roofline`: the dominant kernel of the plain form, its algorithmic bytes
     per launch over its average launch time from HIP events on the state's
     stream, against the 8 TB/s HBM peak;
+  * `brick33`: the 33-qubit random brickwork (depth 24, {U, CX, CZ, CCX},
+    1242 gates) at its real size: plain complex128 (128 GiB) for one step,
+    fused complex64 k = 5 blocks (64 GiB, resident beside it) for
+    min(steps, 5) steps, 1-F and max |amplitude error| between them, and
+    exact runs in complex128 for one step (--no-brick33 skips it);
   * `cpu_baseline`: ket's own simulator (the reference library built from its
-    sources, oracle/_ref) timed on this host on a bounded sample.
+    sources, oracle/_ref) timed on this host on a bounded sample of the
+    workload's gates at its real size, weighted by its gate mix, on every CPU
+    this process may use; `host` names the CPU model and counts.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload qft30]
 """
@@ -114,22 +121,77 @@ def dominant_kernel(records):
     return kind, d, avg_ms, avg_bytes, by
 
 
-def cpu_baseline(circ, threads, short=4, long_=24):
-    """ket's own run() (reference library, oracle/_ref) on prefixes of the
-    workload at its real size: rate = extra gates / extra time between a short
-    and a longer prefix, so the 2^n allocation is not charged per gate."""
+def host_cpus():
+    """This host's CPU model, the logical CPUs it reports, and how many of them
+    this process may use (its affinity mask, capped by a cgroup CPU quota)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            if q != "max":
+                quota = max(1, int(-(-int(q) // int(period))))
+    except (OSError, ValueError):
+        pass
+    usable = min(allowed, quota) if quota else allowed
+    return {"model": model, "reported": os.cpu_count(), "affinity": allowed,
+            "cgroup_quota": quota, "usable": usable}
+
+
+def cpu_baseline_mix(circ, threads, per_kind=6, reps=3, seed=0):
+    """ket's own simulator (reference library, oracle/_ref) at the workload's
+    real size, weighted by the workload's gate mix: per_kind gates of each
+    kind drawn from the circuit, each applied `reps` times on one resident
+    state by ket::Stepper (the kernels ket::run applies, without
+    reallocating the 2^n state per gate), median per gate; estimated run time
+    = sum over kinds of (gates of that kind) x (mean median time of that
+    kind). ket's Fuser merges nothing in this circuit (no gate's qubits are a
+    subset of the previous gate's), so run() applies the same passes."""
+    from ket_amd.gates import ARITY, Circuit
     from oracle import oracle as O
     if not O.ref_available():
         return None
-    t_s = min(O.ref_time(circ.prefix(short), threads, 1)["seconds"])
-    t_l = min(O.ref_time(circ.prefix(long_), threads, 1)["seconds"])
-    rate = (long_ - short) / max(t_l - t_s, 1e-9)
-    return {"value": rate, "unit": "gates/s", "cores": threads, "kind": "reference",
-            "sample": (f"ket::run (reference built from source, set_num_threads({threads})) "
-                       f"on the first {short} and first {long_} gates of the same "
-                       f"{circ.n_qubits}-qubit circuit; rate = {long_ - short} gates / "
-                       f"({t_l:.2f}s - {t_s:.2f}s)"),
-            "seconds": [t_s, t_l]}
+    kinds = {}
+    for op in circ.ops:
+        if op[0] in ARITY:
+            kinds.setdefault(op[0], []).append(op)
+    rng = np.random.default_rng(seed)
+    chosen = []
+    for name in sorted(kinds):
+        ops = kinds[name]
+        idx = rng.choice(len(ops), min(per_kind, len(ops)), replace=False)
+        chosen += [ops[i] for i in sorted(idx)]
+    sample = Circuit(circ.n_qubits)
+    for _ in range(reps):
+        sample.ops += chosen
+    r = O.ref_step_time(sample, threads, 1)
+    t = np.array(r["steps"][0]).reshape(reps, len(chosen))
+    med = np.median(t, axis=0)
+    per_kind_s = {}
+    for name in sorted(kinds):
+        sel = [i for i, op in enumerate(chosen) if op[0] == name]
+        per_kind_s[name] = float(np.mean(med[sel]))
+    est = sum(len(kinds[k]) * per_kind_s[k] for k in kinds)
+    gates = circ.n_unitary()
+    counts = {k: len(v) for k, v in sorted(kinds.items())}
+    return {"value": gates / est, "unit": "gates/s", "cores": r["threads"], "kind": "reference",
+            "sample": (f"ket::Stepper of the reference built from source "
+                       f"(set_num_threads({threads})) on the {circ.n_qubits}-qubit state: "
+                       f"{len(chosen)} gates of the circuit ({per_kind} per kind), each "
+                       f"applied {reps}x, median per gate; run time estimated as "
+                       f"sum_kind count x mean median = {est:.1f} s for {gates} gates "
+                       f"({counts})"),
+            "estimated_run_s": est, "per_gate_s": per_kind_s, "gate_counts": counts,
+            "setup_s": r["setup"], "measured_s": float(t.sum()), "threads": r["threads"]}
 
 
 def emit(out):
@@ -137,57 +199,68 @@ def emit(out):
     return 0
 
 
-def bench_brick33(args, circ, desc):
-    """Config 3 at its real size: fused complex64 (k = 5, 64 GiB) against the
-    plain complex128 form (128 GiB), both resident; 1-F between them."""
+def brick33_forms(circ, fused_steps, fused_warmup):
+    """Config 3 at its real size: plain complex128 (128 GiB, one step), fused
+    complex64 k = 5 blocks on MFMA (64 GiB, resident beside it), 1-F and
+    max |amplitude error| between the two, and the same complex128
+    arithmetic as exact runs (one step, equal to the plain form bit for bit).
+    Each form's state is reset to |0...0> inside its timed steps."""
     n, gates = circ.n_qubits, circ.n_unitary()
-    out = {"metric": METRIC}
-    fused = N.StateVector(n, N.C64)
-    log(f"[bench] brick33 fused complex64 k=5: {gates} gates")
-    w, e, st = time_steps(fused, circ, N.FUSE_BLOCKS, 5, args.steps, args.warmup)
-    ps_f = w / args.steps
-    log(f"[bench] fused: {ps_f:.2f} s/step, {gates / ps_f:.1f} gates/s, {st['launches']} launches, "
-        f"{st['alg_bytes'] / ps_f / 1e9:.0f} GB/s")
+    res = {"workload": "brick33: 33-qubit random brickwork, depth 24, {U,CX,CZ,CCX}",
+           "n_qubits": n, "gates": gates}
     plain = N.StateVector(n, N.C128)
-    log("[bench] plain complex128 (one step)")
+    log(f"[bench] brick33 plain complex128 (one step, {gates} gates)")
     wp, ep, stp = time_steps(plain, circ, N.FUSE_NONE, 5, 1, 0)
+    gbps = stp["alg_bytes"] / wp / 1e9
+    log(f"[bench] brick33 plain: {wp:.2f} s, {gates / wp:.1f} gates/s, {gbps:.0f} GB/s "
+        f"({gbps / HBM_PEAK_GBPS:.1%} of 8 TB/s)")
+    res["plain_c128"] = {"s_per_step": wp, "gates_per_s": gates / wp, "alg_GBps": gbps,
+                         "roofline_frac": gbps / HBM_PEAK_GBPS, "launches": stp["launches"],
+                         "alg_bytes": stp["alg_bytes"]}
+    fused = N.StateVector(n, N.C64)
+    log(f"[bench] brick33 fused complex64 k=5: warmup {fused_warmup}, steps {fused_steps}")
+    w, e, st = time_steps(fused, circ, N.FUSE_BLOCKS, 5, fused_steps, fused_warmup)
+    ps_f = w / fused_steps
     f, dmax = fused.fidelity(plain)
-    log(f"[bench] plain: {wp:.2f} s, {gates / wp:.1f} gates/s, {stp['alg_bytes'] / wp / 1e9:.0f} GB/s "
-        f"({stp['alg_bytes'] / wp / 1e9 / HBM_PEAK_GBPS:.1%}); 1-F={1 - f:.2e} max|d|={dmax:.2e}")
-    out.update({
-        "value": gates / wp, "unit": "gates/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": wp * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128", "data": "synthetic",
-        "config": {"workload": desc, "n_qubits": n, "gates": gates, "form": "plain per-gate complex128"},
-        "hbm_GBps_algorithmic": stp["alg_bytes"] / wp / 1e9,
-        "roofline_frac_plain": stp["alg_bytes"] / wp / 1e9 / HBM_PEAK_GBPS,
-        "fused": {"c64_k5": {"s_per_step": ps_f, "gates_per_s": gates / ps_f,
-                             "alg_GBps": st["alg_bytes"] / ps_f / 1e9, "launches": st["launches"],
-                             "blocks": st["blocks"], "one_minus_F": 1 - f, "max_abs_err": dmax,
-                             "speedup_vs_plain": wp / ps_f}},
-        "device": N.device_info()["name"]})
+    log(f"[bench] brick33 fused: {ps_f:.2f} s/step, {gates / ps_f:.1f} gates/s, "
+        f"{st['launches']} launches, {st['alg_bytes'] / ps_f / 1e9:.0f} GB/s, x{wp / ps_f:.2f} vs "
+        f"plain; 1-F={1 - f:.2e} max|d|={dmax:.2e}")
+    res["fused_c64_k5"] = {"s_per_step": ps_f, "steps": fused_steps, "warmup": fused_warmup,
+                           "gates_per_s": gates / ps_f, "alg_GBps": st["alg_bytes"] / ps_f / 1e9,
+                           "launches": st["launches"], "blocks": st["blocks"],
+                           "plan_s": st["plan_seconds"], "one_minus_F": 1 - f,
+                           "max_abs_err": dmax, "speedup_vs_plain_c128": wp / ps_f}
     fused.close()
-    # Exact runs (per-gate arithmetic on LDS tiles) on a second complex128
-    # state: equal to the plain form bit for bit.
     exact = N.StateVector(n, N.C128)
-    log("[bench] exact runs complex128 (one step)")
+    log("[bench] brick33 exact runs complex128 (one step)")
     wx, ex, stx = time_steps(exact, circ, N.FUSE_EXACT, 0, 1, 0)
     fx, dx = exact.fidelity(plain)
-    log(f"[bench] exact runs: {wx:.2f} s, {gates / wx:.1f} gates/s, {stx['launches']} launches, "
-        f"x{wp / wx:.2f} vs plain, max|d| vs plain {dx:.1e}")
-    out["exact_runs"] = {"c128": {"s_per_step": wx, "gates_per_s": gates / wx,
-                                  "launches": stx["launches"], "max_abs_diff_vs_plain": dx,
-                                  "speedup_vs_plain": wp / wx}}
+    log(f"[bench] brick33 exact runs: {wx:.2f} s, {gates / wx:.1f} gates/s, {stx['launches']} "
+        f"launches, x{wp / wx:.2f} vs plain, max|d| vs plain {dx:.1e}")
+    res["exact_runs_c128"] = {"s_per_step": wx, "gates_per_s": gates / wx,
+                              "launches": stx["launches"], "max_abs_diff_vs_plain": dx,
+                              "speedup_vs_plain": wp / wx}
     exact.close()
     plain.close()
+    return res
+
+
+def bench_brick33(args, circ, desc):
+    """Config 3 alone: the brick33 forms with --steps fused steps."""
+    res = brick33_forms(circ, args.steps, args.warmup)
+    p = res["plain_c128"]
+    out = {"metric": METRIC, "value": p["gates_per_s"], "unit": "gates/s", "n_gpus": 1,
+           "steps": 1, "warmup": 0, "ms_per_step": p["s_per_step"] * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic",
+           "config": {"workload": desc, "n_qubits": circ.n_qubits, "gates": circ.n_unitary(),
+                      "form": "plain per-gate complex128"},
+           "brick33": res, "device": N.device_info()["name"]}
     if not args.no_cpu:
-        # The reference on the real 2^33 shape (128 GiB of host memory): the
-        # first 2 and first 6 gates of the same circuit.
-        log(f"[bench] cpu baseline with {args.cpu_threads} threads (33-qubit prefixes)")
-        cpu = cpu_baseline(circ, args.cpu_threads, short=2, long_=6)
-        out["cpu_baseline"] = cpu
-        if cpu:
-            log(f"[bench] cpu: {cpu['value']:.3f} gates/s ({cpu['sample']})")
+        threads = host_cpus()["usable"]
+        log(f"[bench] cpu baseline with {threads} threads (33-qubit state)")
+        out["cpu_baseline"] = cpu_baseline_mix(circ, threads, per_kind=3, reps=3)
+        out["host"] = host_cpus()
     return emit(out)
 
 
@@ -375,7 +448,10 @@ def main():
                     help="planner train: where the trained cost model is written "
                          "(the shipped one is ket_amd/cost_model.npz)")
     ap.add_argument("--no-fused", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: every CPU this process may use)")
+    ap.add_argument("--no-brick33", action="store_true",
+                    help="default run: skip the 33-qubit forms (about 90 s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this backend has no multi-GPU path "
@@ -453,11 +529,18 @@ def main():
             sv.close()
     plain.close()
 
+    brick = None
+    if not args.no_brick33 and args.workload == "qft30":
+        brick = brick33_forms(W.brickwork_mixed(33, 24, seed=0), min(args.steps, 5), 1)
+
     cpu = None
+    host = host_cpus()
     if not args.no_cpu:
-        log(f"[bench] cpu baseline with {args.cpu_threads} threads")
+        threads = args.cpu_threads or host["usable"]
+        log(f"[bench] cpu baseline with {threads} threads ({host['model']}, "
+            f"{host['reported']} logical CPUs reported, {host['usable']} usable)")
         try:
-            cpu = cpu_baseline(circ, args.cpu_threads)
+            cpu = cpu_baseline_mix(circ, threads)
         except (subprocess.CalledProcessError, OSError) as e:
             log(f"[bench] cpu baseline failed: {e}")
         if cpu:
@@ -472,7 +555,9 @@ def main():
         "hbm_GBps_algorithmic": gbps,
         "roofline": roofline,
         "fused": fused,
+        "brick33": brick,
         "cpu_baseline": cpu,
+        "host": host,
         "device": info["name"],
     })
     print(json.dumps(out), flush=True)

@@ -196,3 +196,13 @@ def ref_time(circuit, threads: int, reps: int) -> dict:
         with open(cpath, "w") as f:
             f.write(circuit.to_text())
         return json.loads(_ref(["time", cpath, threads, reps]).stdout)
+
+
+def ref_step_time(circuit, threads: int, reps: int) -> dict:
+    """The reference's Stepper over `circuit` on one resident state: setup
+    seconds and per-step seconds (reps lists), tests/cpp/ket_driver.cpp."""
+    with tempfile.TemporaryDirectory() as d:
+        cpath = os.path.join(d, "c.txt")
+        with open(cpath, "w") as f:
+            f.write(circuit.to_text())
+        return json.loads(_ref(["step-time", cpath, threads, reps]).stdout)

@@ -12,6 +12,9 @@
 // Modes:
 //   run <circ> <out.bin> [threads]     final state as raw complex128 (re, im)
 //   time <circ> <threads> <reps>       JSON with the wall time of each run()
+//   step-time <circ> <threads> <reps>  JSON: the time to build a Stepper (the
+//                                      ground state) and of every step, reps
+//                                      passes over the circuit on one state
 //   sample <circ> <seed> <shots> <out.bin>
 //       one simulate() then <shots> Simulation::sample(rng) calls on one
 //       std::mt19937{seed}; writes each shot's classical register packed into
@@ -41,6 +44,7 @@
 #include <ket/parallelism.hpp>
 #include <ket/simulator.hpp>
 #include <ket/state.hpp>
+#include <ket/stepper.hpp>
 
 namespace {
 
@@ -124,7 +128,7 @@ void write_u64(const std::vector<std::uint64_t>& v, const std::string& path) {
 }
 
 int usage() {
-  std::cerr << "usage: ket_driver run|time|sample|sample-calls|expval|stab-sample|"
+  std::cerr << "usage: ket_driver run|time|step-time|sample|sample-calls|expval|stab-sample|"
                "stab-expval|uniforms ...\n";
   return 2;
 }
@@ -157,6 +161,33 @@ int main(int argc, char** argv) {
                   << std::chrono::duration<double>(t1 - t0).count();
       }
       std::cout << "]}\n";
+      return 0;
+    }
+    if (mode == "step-time" && argc >= 5) {
+      // Per-gate times on one resident state: Stepper applies each DAG node
+      // with the same kernels run() uses, without reallocating the state.
+      ket::set_num_threads(static_cast<unsigned>(std::stoul(argv[3])));
+      const int reps = std::stoi(argv[4]);
+      const ket::Circuit c = parse_circuit(argv[2]);
+      const auto t0 = std::chrono::steady_clock::now();
+      ket::Stepper st(c);
+      const auto t1 = std::chrono::steady_clock::now();
+      std::cout << "{\"threads\": " << ket::num_threads()
+                << ", \"setup\": " << std::chrono::duration<double>(t1 - t0).count()
+                << ", \"steps\": [";
+      for (int r = 0; r < reps; ++r) {
+        std::cout << (r ? "], [" : "[");
+        st.seek(0);
+        for (std::size_t k = 0; k < st.size(); ++k) {
+          const auto a = std::chrono::steady_clock::now();
+          st.step();
+          const auto b = std::chrono::steady_clock::now();
+          std::cout << (k ? ", " : "") << std::chrono::duration<double>(b - a).count();
+        }
+      }
+      volatile double sink = st.state()[0].real();
+      (void)sink;
+      std::cout << "]]}\n";
       return 0;
     }
     if ((mode == "sample" || mode == "sample-calls" || mode == "stab-sample") && argc >= 6) {
