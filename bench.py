@@ -264,6 +264,43 @@ def bench_brick33(args, circ, desc):
     return emit(out)
 
 
+FORMS = {  # name: (precision, fusion, block width)
+    "plain": (N.C128, N.FUSE_NONE, 0),
+    "c128_exact_runs": (N.C128, N.FUSE_EXACT, 0),
+    "c128_ket_fuser_runs": (N.C128, N.FUSE_REFERENCE, 0),
+    "c128_k4": (N.C128, N.FUSE_BLOCKS, 4),
+    "c64_exact_runs": (N.C64, N.FUSE_EXACT, 0),
+    "c64_k4": (N.C64, N.FUSE_BLOCKS, 4),
+    "c64_k5": (N.C64, N.FUSE_BLOCKS, 5),
+}
+
+
+def bench_form(args, circ, desc):
+    """One form of a workload on its own (for profiling): --steps timed steps
+    after --warmup, then one pass with a HIP event pair around every launch."""
+    prec, fusion, k = FORMS[args.form]
+    n, gates = circ.n_qubits, circ.n_unitary()
+    sv = N.StateVector(n, prec)
+    w, e, st = time_steps(sv, circ, fusion, k, args.steps, args.warmup)
+    ps = w / args.steps
+    sv.set_basis(0)
+    recs = sv.profile_circuit(circ, fusion, k)
+    sv.close()
+    launches = [{"kind": r["kind"], "k": r["k"], "n_gates": r["n_gates"], "ms": r["ms"],
+                 "alg_bytes": r["alg_bytes"]} for r in recs]
+    log(f"[bench] {args.form}: {ps * 1e3:.1f} ms/step, {gates / ps:.1f} gates/s, "
+        f"{st['launches']} launches")
+    for i, r in enumerate(launches):
+        log(f"[bench]   launch {i}: {r['kind']} k={r['k']} gates={r['n_gates']} {r['ms']:.3f} ms "
+            f"{r['alg_bytes'] / (r['ms'] / 1e3) / 1e9:.0f} GB/s")
+    return emit({"metric": METRIC, "value": gates / ps, "unit": "gates/s", "n_gpus": 1,
+                 "steps": args.steps, "warmup": args.warmup, "ms_per_step": ps * 1e3,
+                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                 "dtype": "c128" if prec == N.C128 else "c64", "data": "synthetic",
+                 "config": {"workload": desc, "form": args.form, "n_qubits": n, "gates": gates},
+                 "launches": launches, "device": N.device_info()["name"]})
+
+
 def bench_hea(args):
     """Config 4: 4096 instances, widths U{10..22}, one sweep per step with a
     new parameter binding (two angle sets alternate), complex128 and complex64,
@@ -450,6 +487,8 @@ def main():
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (0: every CPU this process may use)")
+    ap.add_argument("--form", choices=sorted(FORMS),
+                    help="run only this form of the workload (profiling)")
     ap.add_argument("--no-brick33", action="store_true",
                     help="default run: skip the 33-qubit forms (about 90 s)")
     args = ap.parse_args()
@@ -474,6 +513,8 @@ def main():
         f"HBM={info['hbm_bytes'] / 2**30:.0f} GiB; workload {desc}")
 
     out = {"metric": METRIC}
+    if args.form:
+        return bench_form(args, circ, desc)
     if args.workload == "brick33":
         return bench_brick33(args, circ, desc)
     copy_gbps = max(N.calibrate_copy(4 << 30, 5, nt) for nt in (0, 1))

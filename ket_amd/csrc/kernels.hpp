@@ -64,29 +64,75 @@ hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
 
 // Exact runs: consecutive per-gate passes applied to LDS tiles of 2^TB
 // amplitudes (TB = run_tile_bits), each with its own kernel's arithmetic, in
-// order, so a run equals its passes bit for bit. A record is a RunRecord,
-// then n_ops RunOps, then the 4^k complex128 matrices of its exact-matrix
-// ops. Tile positions P (pmask) hold every bit an op moves; local bit j of a
-// tile index is bit j-th-lowest of P.
+// order, so a run equals its passes bit for bit. Tile positions P (pmask)
+// hold every bit an op moves; local bit j of a tile index is the j-th-lowest
+// bit of P.
+//
+// The ops are applied in registers: 256 threads each hold RPT = 2^(TB-8)
+// amplitudes of the tile. A segment is a stretch of consecutive ops whose
+// moved bits (pair partners, matrix bits) are all register bits of one
+// layout: register bit r of a thread's amplitude index is tile bit rpos[r],
+// thread bit i is tile bit tpos[i]. Within a segment no amplitude moves
+// between threads, so ops need no barrier; between segments each thread
+// writes its amplitudes back to the LDS slots it read them from, and the
+// next layout reads them after one barrier. LDS slot of tile index t =
+// t ^ (XOR over the set bits b >= RB of t of slot bit (b mod RB)), RB =
+// log2(128 / amplitude bytes): a fixed XOR swizzle under which lanes whose
+// tile bits have distinct residues mod RB reach distinct slots of a
+// 128-byte LDS row. A record is a RunRecord, n_ops RunOps, n_segs RunSegs,
+// then the 4^k complex128 matrices of its exact-matrix ops.
 enum RunKind { RUN_DENSE = 0, RUN_CH = 1, RUN_SWAP = 2, RUN_PHASE = 3, RUN_MATRIX = 4 };
 struct RunOp {
+  // read by k_run_tile: the op in its segment's register layout (matrix
+  // ops: matrix bit b is register bit b, so the reg_* fields stay 0)
   int32_t kind;
   int32_t npos;        // local positions in pos (matrix: its width k)
+  uint64_t out_mask;   // bits outside P that must all be set (outside controls)
+  uint32_t reg_lane;   // tile bits held by thread bits that must be set
+  uint32_t reg_cond;   // register-index bits that must be set
+  uint32_t reg_part;   // register-index bits a pair's two amplitudes differ in
+  uint32_t reg_lo;     // their values in the pair's lower amplitude
+  uint32_t reg_mask;   // registers j the op applies to: phases, j holding
+                       // reg_cond; pairs, the lower amplitudes (j's reg_part
+                       // bits read reg_lo) holding reg_cond
+  uint32_t stretch;    // phase ops: consecutive phase ops of the segment from
+                       // this one on (this one included)
+  double m[8];         // 2x2 row-major (re, im); CH: m[0]; phase: m[0], m[1]
+  uint32_t mat_offset; // matrix ops: bytes from the record start
+  int32_t pad;
+  // the op's local form (planner, CPU interpreter)
   int32_t pos[4];      // local positions, ascending (zero bits inserted there)
   int32_t mpos[4];     // matrix ops: local position of matrix index bit b
   uint32_t set;        // local bits set in the lower index of a pair / phase
   uint32_t partner;    // local bits a pair's two indices differ in
-  uint64_t out_mask;   // bits outside P that must all be set (outside controls)
-  double m[8];         // 2x2 row-major (re, im); CH: m[0]; phase: m[0], m[1]
-  uint32_t mat_offset; // matrix ops: bytes from the record start
+};
+constexpr int kRunMaxRegs = 16;  // amplitudes per thread (complex64: 2^(12-8))
+struct RunSeg {
+  int32_t first_op;
+  int32_t n_ops;
+  uint32_t rmask;                 // tile bits held in registers
   int32_t pad;
+  uint16_t roff[kRunMaxRegs];     // register j: its tile-index bits
+  uint16_t rslot[kRunMaxRegs];    // ... and their slot bits
+  uint16_t tbit[8];               // thread bit i: its tile-index bit
+  uint16_t tslot[8];              // ... and its slot bits
 };
 struct RunRecord {
   uint64_t pmask;
   int32_t n_ops;
-  int32_t pad;
+  int32_t n_segs;
+  uint16_t sbit[12];  // slot bits of tile bit b
+  int32_t pad[2];
 };
-static_assert(sizeof(RunOp) == 128 && sizeof(RunRecord) == 16, "run record layout");
+static_assert(sizeof(RunOp) == 152 && sizeof(RunSeg) == 112 && sizeof(RunRecord) == 48,
+              "run record layout");
+// Slot bits of tile index t under the runs' LDS swizzle.
+inline uint32_t run_slot(uint32_t t, int row_bits) {
+  uint32_t x = t & ((1u << row_bits) - 1);
+  for (int b = row_bits; (t >> b) != 0; ++b)
+    if ((t >> b) & 1) x ^= 1u << (b % row_bits);
+  return (t & ~((1u << row_bits) - 1)) | x;
+}
 inline int run_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 // n >= run_tile_bits(precision); rec_dev: the record in device memory.
 hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev);

@@ -308,6 +308,142 @@ bool local_op(const Plan& plan, const Op& o, uint64_t P, ketk::RunOp* r,
   return true;
 }
 
+// ---- register layouts of a run (ketk::RunSeg) ----
+
+// Tile bits an op moves between amplitudes: pair partners, matrix bits.
+uint32_t run_moved(const ketk::RunOp& r) {
+  if (r.kind == ketk::RUN_PHASE) return 0;
+  if (r.kind == ketk::RUN_MATRIX) {
+    uint32_t m = 0;
+    for (int b = 0; b < r.npos; ++b) m |= 1u << r.mpos[b];
+    return m;
+  }
+  return r.partner;
+}
+
+// Tile bits an op tests (controls, phase bits), without the partner bits.
+uint32_t run_cond(const ketk::RunOp& r) {
+  return r.kind == ketk::RUN_MATRIX ? 0u : r.set & ~r.partner;
+}
+
+// Bits of tile mask x at the register positions rpos, packed low.
+uint32_t to_regs(uint32_t x, const int* rpos, int nr) {
+  uint32_t r = 0;
+  for (int b = 0; b < nr; ++b)
+    if ((x >> rpos[b]) & 1) r |= 1u << b;
+  return r;
+}
+
+// Splits the ops of a run into segments of one register layout each and
+// fills their register form (RunOp::reg_*) and tables. A segment grows while
+// the bits its ops move fit in the nr register bits and its exact-matrix ops
+// share one bit order (matrix bit b = register bit b). Its register bits are
+// those moved bits plus the tile bits its ops test most often (a test on a
+// register bit skips whole registers); of the other bits the most tested
+// become the two wave bits (thread bits 6, 7: skipped per wave), and the
+// thread bits 0..rb-1, whose lanes share LDS banks in one access, take the
+// least tested bits of distinct residues mod rb where they exist (distinct
+// rows of the swizzle).
+std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, int row_bits) {
+  const int nr = tb - 8;
+  std::vector<ketk::RunSeg> segs;
+  std::size_t i = 0;
+  while (i < ops.size()) {
+    uint32_t need = 0;
+    std::vector<int> order;  // the segment's matrix bit order
+    std::size_t j = i;
+    for (; j < ops.size(); ++j) {
+      const uint32_t m = need | run_moved(ops[j]);
+      bool ok = __builtin_popcount(m) <= nr;
+      std::vector<int> o;
+      if (ops[j].kind == ketk::RUN_MATRIX) {
+        o.assign(ops[j].mpos, ops[j].mpos + ops[j].npos);
+        if (!order.empty() && o != order) ok = false;
+      }
+      if (!ok && j > i) break;
+      need = m;
+      if (!o.empty()) order = o;
+    }
+    int use[16] = {};
+    for (std::size_t t = i; t < j; ++t)
+      for (uint32_t c = run_cond(ops[t]); c; c &= c - 1) ++use[__builtin_ctz(c)];
+    int rpos[4], n_r = 0;
+    uint32_t rmask = 0;
+    auto take = [&](int b) {
+      if (n_r < nr && !((rmask >> b) & 1)) {
+        rpos[n_r++] = b;
+        rmask |= 1u << b;
+      }
+    };
+    for (int b : order) take(b);
+    for (int b = 0; b < tb; ++b)
+      if ((need >> b) & 1) take(b);
+    while (n_r < nr) {  // the most tested other bits
+      int best = -1;
+      for (int b = 0; b < tb; ++b)
+        if (!((rmask >> b) & 1) && (best < 0 || use[b] > use[best])) best = b;
+      take(best);
+    }
+    std::vector<int> rest;  // thread bits, least tested first
+    for (int b = 0; b < tb; ++b)
+      if (!((rmask >> b) & 1)) rest.push_back(b);
+    std::stable_sort(rest.begin(), rest.end(), [&](int a, int b) { return use[a] < use[b]; });
+    int tpos[8];
+    bool used[16] = {};
+    bool res[8] = {};
+    int nt = 0;
+    for (int b : rest)  // bank lanes: distinct residues, least tested first
+      if (nt < row_bits && !res[b % row_bits]) {
+        res[b % row_bits] = true;
+        used[b] = true;
+        tpos[nt++] = b;
+      }
+    for (int b : rest)
+      if (nt < row_bits && !used[b]) {
+        used[b] = true;
+        tpos[nt++] = b;
+      }
+    for (int b : rest)
+      if (!used[b]) tpos[nt++] = b;  // ascending use: the last two are the wave bits
+    ketk::RunSeg sg{};
+    sg.first_op = (int32_t)i;
+    sg.n_ops = (int32_t)(j - i);
+    sg.rmask = rmask;
+    for (int r = 0; r < (1 << nr); ++r) {
+      uint32_t t = 0;
+      for (int b = 0; b < nr; ++b)
+        if ((r >> b) & 1) t |= 1u << rpos[b];
+      sg.roff[r] = (uint16_t)t;
+      sg.rslot[r] = (uint16_t)ketk::run_slot(t, row_bits);
+    }
+    for (int b = 0; b < 8; ++b) {
+      sg.tbit[b] = (uint16_t)(1u << tpos[b]);
+      sg.tslot[b] = (uint16_t)ketk::run_slot(1u << tpos[b], row_bits);
+    }
+    for (std::size_t t = j; t-- > i;)  // phase stretches, counted from their ends
+      ops[t].stretch = ops[t].kind != ketk::RUN_PHASE ? 0
+                       : (t + 1 < j && ops[t + 1].kind == ketk::RUN_PHASE)
+                           ? ops[t + 1].stretch + 1
+                           : 1;
+    for (std::size_t t = i; t < j; ++t) {
+      ketk::RunOp& r = ops[t];
+      if (r.kind == ketk::RUN_MATRIX) continue;
+      const uint32_t c = run_cond(r);
+      r.reg_cond = to_regs(c & rmask, rpos, nr);
+      r.reg_lane = c & ~rmask;
+      r.reg_part = to_regs(r.partner, rpos, nr);
+      r.reg_lo = to_regs(r.set & r.partner, rpos, nr);
+      r.reg_mask = 0;
+      for (uint32_t q = 0; q < (1u << nr); ++q)
+        if ((q & r.reg_cond) == r.reg_cond && (q & r.reg_part) == r.reg_lo)
+          r.reg_mask |= 1u << q;
+    }
+    segs.push_back(sg);
+    i = j;
+  }
+  return segs;
+}
+
 }  // namespace
 
 void group_runs(Plan& plan, int n, int precision) {
@@ -353,7 +489,10 @@ void group_runs(Plan& plan, int n, int precision) {
       ++i;
       continue;
     }
-    const std::size_t head = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp);
+    const int row_bits = line_bits(precision);
+    const std::vector<ketk::RunSeg> segs = run_segments(ops, tb, row_bits);
+    const std::size_t head = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp) +
+                             segs.size() * sizeof(ketk::RunSeg);
     for (auto& r : ops)
       if (r.kind == ketk::RUN_MATRIX) r.mat_offset += (uint32_t)head;
     const std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
@@ -361,9 +500,14 @@ void group_runs(Plan& plan, int n, int precision) {
     ketk::RunRecord h{};
     h.pmask = P;
     h.n_ops = (int)ops.size();
-    std::memcpy(plan.pool.data() + off, &h, sizeof h);
-    std::memcpy(plan.pool.data() + off + sizeof h, ops.data(), ops.size() * sizeof(ketk::RunOp));
-    if (!mats.empty()) std::memcpy(plan.pool.data() + off + head, mats.data(), mats.size());
+    h.n_segs = (int)segs.size();
+    for (int b = 0; b < tb; ++b) h.sbit[b] = (uint16_t)ketk::run_slot(1u << b, row_bits);
+    unsigned char* dst = plan.pool.data() + off;
+    std::memcpy(dst, &h, sizeof h);
+    std::memcpy(dst + sizeof h, ops.data(), ops.size() * sizeof(ketk::RunOp));
+    std::memcpy(dst + sizeof h + ops.size() * sizeof(ketk::RunOp), segs.data(),
+                segs.size() * sizeof(ketk::RunSeg));
+    if (!mats.empty()) std::memcpy(dst + head, mats.data(), mats.size());
     Op o;
     o.kind = Kind::Run;
     o.pool_offset = off;

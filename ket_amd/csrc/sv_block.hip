@@ -888,36 +888,134 @@ hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uin
 
 // ---- exact runs: consecutive per-gate passes on one LDS tile ----
 // Each op repeats its per-gate kernel's arithmetic (k_pair, k_phase,
-// k_matrix_exact in sv_gates.hip) on the tile's amplitudes; ops run in plan
-// order with a barrier between them, so every amplitude sees the same
-// operations on the same values as with one pass per op.
+// k_matrix_exact in sv_gates.hip) on the tile's amplitudes, in plan order,
+// so every amplitude sees the same operations on the same values as with one
+// pass per op. The tile is loaded into LDS in index order; then each segment
+// of ops (RunSeg) reads the thread's RPT amplitudes of its layout into
+// registers, applies its ops there (partners and matrix bits are register
+// bits, so no amplitude moves between threads and no barrier is needed) and
+// writes them back to the slots it read; one barrier separates segments.
 
-// One exact-matrix op of width K: per group, acc = 0; acc += M[r][c] * in[c]
-// for c ascending (the matrix converted to T element by element, as
-// k_matrix_exact does).
-template <class T, int TB, int K>
-__device__ __forceinline__ void run_matrix(ketd::cplx<T>* L, const RunOp& op,
-                                           const unsigned char* rec) {
-  constexpr int D = 1 << K;
-  const auto* mat = reinterpret_cast<const double2*>(rec + op.mat_offset);
-  uint32_t off[D];
+// Pairs of one op whose register-index bits PR differ: (v[j], v[j ^ PR])
+// for the lower amplitudes j of rm (RunOp::reg_mask). LANE: lanes where
+// `keep` is set leave their amplitudes unchanged (a select per component:
+// no lane branches off, and a select of scalars, unlike one of whole
+// structs, keeps v in registers).
+template <class T, int RPT, int PR, bool LANE>
+__device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uint32_t rm,
+                                          const double* m, bool keep) {
+  constexpr uint32_t kAll = [] {  // every lower amplitude: no register condition
+    uint32_t x = 0;
+    for (int j = 0; j < RPT; ++j)
+      if ((j & PR) == 0) x |= 1u << j;
+    return x;
+  }();
+  if (kind == RUN_DENSE && rm == kAll) {  // straight block
+    const ketd::cplx<T> m00 = ketd::mk<T>((T)m[0], (T)m[1]);
+    const ketd::cplx<T> m01 = ketd::mk<T>((T)m[2], (T)m[3]);
+    const ketd::cplx<T> m10 = ketd::mk<T>((T)m[4], (T)m[5]);
+    const ketd::cplx<T> m11 = ketd::mk<T>((T)m[6], (T)m[7]);
 #pragma unroll
-  for (int e = 0; e < D; ++e) {
-    uint32_t o = 0;
+    for (int j = 0; j < RPT; ++j) {
+      if ((j & PR) != 0) continue;  // compile time
+      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
+      ketd::cplx<T> x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
+      ketd::cplx<T> y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
+      if constexpr (LANE) {
+        x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
+        y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
+      }
+      v[j] = x;
+      v[j ^ PR] = y;
+      if (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);  // one pair in flight
+    }
+  } else if (kind == RUN_DENSE) {
+    const ketd::cplx<T> m00 = ketd::mk<T>((T)m[0], (T)m[1]);
+    const ketd::cplx<T> m01 = ketd::mk<T>((T)m[2], (T)m[3]);
+    const ketd::cplx<T> m10 = ketd::mk<T>((T)m[4], (T)m[5]);
+    const ketd::cplx<T> m11 = ketd::mk<T>((T)m[6], (T)m[7]);
 #pragma unroll
-    for (int b = 0; b < K; ++b)
-      if ((e >> b) & 1) o |= 1u << op.mpos[b];
-    off[e] = o;
+    for (int j = 0; j < RPT; ++j) {
+      if (!((rm >> j) & 1)) continue;  // uniform
+      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
+      ketd::cplx<T> x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
+      ketd::cplx<T> y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
+      if constexpr (LANE) {
+        x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
+        y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
+      }
+      v[j] = x;
+      v[j ^ PR] = y;
+    }
+  } else if (kind == RUN_CH) {
+    const T r = (T)m[0];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      if (!((rm >> j) & 1)) continue;
+      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
+      ketd::cplx<T> x = ketd::mk<T>((a.x + b.x) * r, (a.y + b.y) * r);
+      ketd::cplx<T> y = ketd::mk<T>((a.x - b.x) * r, (a.y - b.y) * r);
+      if constexpr (LANE) {
+        x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
+        y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
+      }
+      v[j] = x;
+      v[j ^ PR] = y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      if (!((rm >> j) & 1)) continue;
+      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
+      if constexpr (LANE) {
+        v[j] = ketd::mk<T>(keep ? a.x : b.x, keep ? a.y : b.y);
+        v[j ^ PR] = ketd::mk<T>(keep ? b.x : a.x, keep ? b.y : a.y);
+      } else {
+        v[j] = b;
+        v[j ^ PR] = a;
+      }
+    }
   }
-#pragma unroll 1
-  for (uint32_t g = threadIdx.x; g < (1u << (TB - K)); g += kThreads) {
-    // a compiler-only fence: keeps the matrix reads (scalar loads) inside
-    // the loop rather than hoisted into registers for the whole op
+}
+
+template <class T, int RPT, bool LANE>
+__device__ __forceinline__ void reg_pair_op(ketd::cplx<T> (&v)[RPT], int kind, uint32_t part,
+                                            uint32_t rm, const double* m, bool keep) {
+  switch (part) {  // one or two register bits
+    case 1: reg_pairs<T, RPT, 1, LANE>(v, kind, rm, m, keep); break;
+    case 2: reg_pairs<T, RPT, 2, LANE>(v, kind, rm, m, keep); break;
+    case 4: reg_pairs<T, RPT, 4, LANE>(v, kind, rm, m, keep); break;
+    case 3: reg_pairs<T, RPT, 3, LANE>(v, kind, rm, m, keep); break;
+    case 5: reg_pairs<T, RPT, 5, LANE>(v, kind, rm, m, keep); break;
+    case 6: reg_pairs<T, RPT, 6, LANE>(v, kind, rm, m, keep); break;
+    default:
+      if constexpr (RPT == 16) {
+        switch (part) {
+          case 8: reg_pairs<T, RPT, 8, LANE>(v, kind, rm, m, keep); break;
+          case 9: reg_pairs<T, RPT, 9, LANE>(v, kind, rm, m, keep); break;
+          case 10: reg_pairs<T, RPT, 10, LANE>(v, kind, rm, m, keep); break;
+          case 12: reg_pairs<T, RPT, 12, LANE>(v, kind, rm, m, keep); break;
+          default: break;
+        }
+      }
+      break;
+  }
+}
+
+// An exact-matrix op of width K on register bits 0..K-1: per group, acc = 0;
+// acc += M[r][c] * in[c] for c ascending (the matrix converted to T element
+// by element, as k_matrix_exact does).
+template <class T, int RPT, int K>
+__device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double2* mat) {
+  constexpr int D = 1 << K;
+#pragma unroll
+  for (int g = 0; g < RPT / D; ++g) {
+    // a compiler-only fence: keeps the matrix reads (scalar loads) with their
+    // group rather than held in registers across groups
     __asm__ volatile("" ::: "memory");
-    const uint32_t base = (uint32_t)ketd::insert_zeros(g, K, op.pos);
     ketd::cplx<T> in[D];
 #pragma unroll
-    for (int e = 0; e < D; ++e) in[e] = L[base | off[e]];
+    for (int e = 0; e < D; ++e) in[e] = v[g * D + e];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       ketd::cplx<T> acc = ketd::mk<T>(T(0), T(0));
@@ -926,116 +1024,156 @@ __device__ __forceinline__ void run_matrix(ketd::cplx<T>* L, const RunOp& op,
         acc = ketd::cadd<T>(acc, ketd::cmul<T>(ketd::mk<T>((T)mat[r * D + c].x,
                                                            (T)mat[r * D + c].y),
                                                in[c]));
-      L[base | off[r]] = acc;
+      v[g * D + r] = acc;
     }
   }
 }
 
+// A stretch of consecutive phase ops on the thread's registers, one path
+// per op: whole tiles missing an op's outside bits and registers missing
+// its register condition are skipped by uniform tests (RunOp::reg_mask);
+// lanes missing its lane condition are masked off for the product. Each
+// register's update is a branch around it alone, so the registers stay in
+// place from op to op. base = the tile-index bits of the thread's lanes,
+// fixed = the tile's bits outside P.
+template <class T, int RPT>
+__device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunOp* opp, int count,
+                                           uint32_t base, uint64_t fixed) {
+  for (int q = 0; q < count; ++q, ++opp) {
+    const uint64_t out = opp->out_mask;
+    const uint32_t lane = opp->reg_lane, mask = opp->reg_mask;
+    const uint32_t rm = (fixed & out) == out ? mask : 0u;
+    const ketd::cplx<T> ph = ketd::mk<T>((T)opp->m[0], (T)opp->m[1]);
+    if ((base & lane) == lane) {  // per lane
+#pragma unroll
+      for (int j = 0; j < RPT; ++j)
+        if ((rm >> j) & 1) v[j] = ketd::cmul<T>(v[j], ph);  // uniform
+    }
+  }
+}
+
+// One pair or exact-matrix op on the thread's registers. Every branch here
+// is uniform over the workgroup; the lane condition (reg_lane) is a select.
+template <class T, int RPT>
+__device__ __forceinline__ void reg_op(ketd::cplx<T> (&v)[RPT], const RunOp* opp, uint32_t base,
+                                       uint64_t fixed, const unsigned char* rec) {
+  const int kind = opp->kind;
+  const uint64_t out = opp->out_mask;
+  const uint32_t lane = opp->reg_lane;
+  if ((fixed & out) != out) return;
+  const bool keep = (base & lane) != lane;
+  if (kind == RUN_MATRIX) {
+    const auto* mat = reinterpret_cast<const double2*>(rec + opp->mat_offset);
+    const int k = opp->npos;
+    if (k == 1) reg_matrix<T, RPT, 1>(v, mat);
+    else if (k == 2) reg_matrix<T, RPT, 2>(v, mat);
+    else reg_matrix<T, RPT, 3>(v, mat);
+    return;
+  }
+  const uint32_t part = opp->reg_part, rm = opp->reg_mask;
+  if (lane == 0)
+    reg_pair_op<T, RPT, false>(v, kind, part, rm, opp->m, false);
+  else
+    reg_pair_op<T, RPT, true>(v, kind, part, rm, opp->m, keep);
+}
+
+// Moves a run's tile between HBM and LDS in index order (16-byte accesses,
+// consecutive lanes on consecutive addresses); LDS slots by the runs'
+// swizzle (RunRecord::sbit).
 template <class T, int TB>
-__device__ __forceinline__ void run_op(ketd::cplx<T>* L, const RunOp& op,
-                                       const unsigned char* rec) {
-  switch (op.kind) {
-    case RUN_DENSE:
-    case RUN_CH:
-    case RUN_SWAP: {
-      const ketd::cplx<T> m00 = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
-      const ketd::cplx<T> m01 = ketd::mk<T>((T)op.m[2], (T)op.m[3]);
-      const ketd::cplx<T> m10 = ketd::mk<T>((T)op.m[4], (T)op.m[5]);
-      const ketd::cplx<T> m11 = ketd::mk<T>((T)op.m[6], (T)op.m[7]);
-      for (uint32_t it = threadIdx.x; it < ((1u << TB) >> op.npos); it += kThreads) {
-        const uint32_t lo = (uint32_t)ketd::insert_zeros(it, op.npos, op.pos) | op.set;
-        const uint32_t hi = lo ^ op.partner;
-        const ketd::cplx<T> a = L[lo], b = L[hi];
-        ketd::cplx<T> x, y;
-        if (op.kind == RUN_DENSE) {
-          x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
-          y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
-        } else if (op.kind == RUN_CH) {
-          const T r = m00.x;
-          x = ketd::mk<T>((a.x + b.x) * r, (a.y + b.y) * r);
-          y = ketd::mk<T>((a.x - b.x) * r, (a.y - b.y) * r);
-        } else {
-          x = b;
-          y = a;
-        }
-        L[lo] = x;
-        L[hi] = y;
-      }
-      break;
+struct RunIO {
+  using IO = TileIO<T, TB, kThreads>;
+  using V = typename IO::V;
+  static constexpr int EPA = IO::EPA, S = IO::S, SB = IO::SB, LS = IO::LS;
+  uint64_t g0;
+  uint32_t l0;
+  uint64_t gb[SB];
+  uint32_t lb[SB];
+  __device__ __forceinline__ void init(const RunRecord& h, uint32_t tid) {
+    const uint32_t t = tid * EPA;
+    g0 = deposit(t, h.pmask);
+    l0 = 0;
+    for (int b = 0; b < LS; ++b)
+      if ((t >> b) & 1) l0 ^= h.sbit[b];
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      gb[b] = deposit(uint64_t{1} << (LS + b), h.pmask);
+      lb[b] = h.sbit[LS + b];
     }
-    case RUN_PHASE: {
-      const ketd::cplx<T> ph = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
-      for (uint32_t it = threadIdx.x; it < ((1u << TB) >> op.npos); it += kThreads) {
-        const uint32_t i = (uint32_t)ketd::insert_zeros(it, op.npos, op.pos) | op.set;
-        L[i] = ketd::cmul<T>(L[i], ph);
-      }
-      break;
-    }
-    case RUN_MATRIX:
-      if (op.npos == 1) run_matrix<T, TB, 1>(L, op, rec);
-      else if (op.npos == 2) run_matrix<T, TB, 2>(L, op, rec);
-      else run_matrix<T, TB, 3>(L, op, rec);
-      break;
-    default:
-      break;
   }
-}
-
-// A stretch of consecutive phase ops: diagonal, so each thread applies all
-// of them to the amplitudes it owns (tile indices tid + 256 j) in registers,
-// in op order, with one LDS read and write for the whole stretch and no
-// barrier in between. A condition on tile bits 8 and up holds or fails for
-// a whole register j (a scalar test), so such registers are skipped.
-template <class T, int TB>
-__device__ __forceinline__ void run_phases(ketd::cplx<T>* L, const RunOp* ops, int count,
-                                           uint64_t fixed) {
-  constexpr int PER = (1 << TB) / kThreads;
-  ketd::cplx<T> v[PER];
+  __device__ __forceinline__ uint64_t gaddr(int j) const {
+    uint64_t a = g0;
 #pragma unroll
-  for (int j = 0; j < PER; ++j) v[j] = L[threadIdx.x + j * kThreads];
-  for (int q = 0; q < count; ++q) {
-    const RunOp& op = ops[q];
-    if ((fixed & op.out_mask) != op.out_mask) continue;  // uniform
-    const ketd::cplx<T> ph = ketd::mk<T>((T)op.m[0], (T)op.m[1]);
-    const uint32_t shi = op.set & ~(uint32_t)(kThreads - 1), slo = op.set & (kThreads - 1);
-    const bool lane = (threadIdx.x & slo) == slo;
-#pragma unroll
-    for (int j = 0; j < PER; ++j)
-      if ((((uint32_t)j * kThreads) & shi) == shi && lane) v[j] = ketd::cmul<T>(v[j], ph);
+    for (int b = 0; b < SB; ++b)
+      if ((j >> b) & 1) a |= gb[b];
+    return a;
   }
+  __device__ __forceinline__ uint32_t laddr(int j) const {
+    uint32_t a = l0;
 #pragma unroll
-  for (int j = 0; j < PER; ++j) L[threadIdx.x + j * kThreads] = v[j];
-}
+    for (int b = 0; b < SB; ++b)
+      if ((j >> b) & 1) a ^= lb[b];
+    return a;
+  }
+  __device__ __forceinline__ void load(const ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                       ketd::cplx<T>* L) const {
+    V v[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) v[j] = load_nt(reinterpret_cast<const V*>(s + (fixed | gaddr(j))));
+#pragma unroll
+    for (int j = 0; j < S; ++j) IO::put(L, laddr(j), v[j]);
+  }
+  __device__ __forceinline__ void store(ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                        const ketd::cplx<T>* L) const {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      store_nt(reinterpret_cast<V*>(s + (fixed | gaddr(j))), IO::get(L, laddr(j)));
+  }
+};
 
-// One workgroup per tile (grid-stride): load the tile in index order, apply
-// the ops whose outside controls the tile satisfies, store it back.
+// One workgroup per tile (grid-stride): load the tile, apply each segment's
+// ops in registers, store it back.
 template <class T, int TB>
 __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict__ s,
                                                         uint64_t n_tiles,
                                                         const unsigned char* __restrict__ rec) {
-  using IO = TileIO<T, TB, kThreads>;
+  constexpr int RPT = 1 << (TB - 8);
+  static_assert(RPT <= kRunMaxRegs, "registers per thread");
   __shared__ ketd::cplx<T> L[1 << TB];
   const RunRecord& h = *reinterpret_cast<const RunRecord*>(rec);
   const auto* ops = reinterpret_cast<const RunOp*>(rec + sizeof(RunRecord));
-  TileMap mp{};
-  mp.pmask = h.pmask;
-  mp.nloc = (1u << TB) - 1;  // no swizzle: LDS slot = tile index
-  IO io;
-  io.init(mp, threadIdx.x);
+  const auto* segs = reinterpret_cast<const RunSeg*>(ops + h.n_ops);
+  RunIO<T, TB> io;
+  io.init(h, threadIdx.x);
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint64_t fixed = deposit_outside(tile, h.pmask);
     io.load(s, fixed, L);
-    for (int o = 0; o < h.n_ops; ++o) {
+    for (int sg = 0; sg < h.n_segs; ++sg) {
       __syncthreads();
-      if (ops[o].kind == RUN_PHASE) {
-        int e = o + 1;
-        while (e < h.n_ops && ops[e].kind == RUN_PHASE) ++e;
-        run_phases<T, TB>(L, ops + o, e - o, fixed);
-        o = e - 1;
-        continue;
+      const RunSeg& S = segs[sg];
+      uint32_t base = 0, slot = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {  // branch-free, so the tables load as scalars
+        const uint32_t on = 0u - ((threadIdx.x >> i) & 1u);
+        base |= (uint32_t)S.tbit[i] & on;
+        slot ^= (uint32_t)S.tslot[i] & on;
       }
-      if ((fixed & ops[o].out_mask) != ops[o].out_mask) continue;  // uniform
-      run_op<T, TB>(L, ops[o], rec);
+      ketd::cplx<T> v[RPT];
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) v[j] = L[slot ^ S.rslot[j]];
+      const int end = S.first_op + S.n_ops;
+      for (int o = S.first_op; o < end;) {
+        const RunOp* op = ops + o;
+        if (op->kind == RUN_PHASE) {
+          reg_phases<T, RPT>(v, op, op->stretch, base, fixed);
+          o += op->stretch;
+        } else {
+          reg_op<T, RPT>(v, op, base, fixed, rec);
+          ++o;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) L[slot ^ S.rslot[j]] = v[j];
     }
     __syncthreads();
     io.store(s, fixed, L);

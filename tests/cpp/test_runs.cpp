@@ -1,15 +1,19 @@
 // CPU check of exact runs (no GPU): a circuit is planned one pass per gate
 // and with consecutive passes grouped into runs (ketplan::group_runs); both
 // plans are executed here, the per-gate passes with the arithmetic of
-// k_pair / k_phase / k_matrix_exact and the runs with k_run_tile's tile loop
-// over their records, and the two states must be equal bit for bit.
+// k_pair / k_phase / k_matrix_exact and the runs with k_run_tile's walk over
+// their records (tile loads through the LDS swizzle, each segment's layout
+// thread by thread, every op in its register form), and the two states must
+// be equal bit for bit. Every record's layouts are checked first
+// (check_record).
 //
 //   test_runs <records.bin> <n> <precision 0|1> <mode: none | reference | features>
 //             [run_low_bits]
 //
 // mode none compares KET_FUSE_NONE with KET_FUSE_EXACT; mode reference
 // compares KET_FUSE_REFERENCE with exact runs off and on. Prints
-// "ok <passes> <runs> <passes in runs>" or the first difference. Mode
+// "ok <passes> <runs> <passes in runs> <LDS bank-row conflicts>" or the
+// first difference. Mode
 // features compares the learned planner's feature rows from the packer with
 // and without its matrix-free recording mode.
 #include <cstdint>
@@ -113,35 +117,200 @@ void matrix(C<T>* s, uint64_t size, int k, const int* sorted, const uint64_t* of
   }
 }
 
+[[noreturn]] void bad_record(const char* what) {
+  std::printf("bad run record: %s\n", what);
+  std::exit(1);
+}
+
+// k_run_tile's register form of one op (reg_op in sv_block.hip).
+template <class T>
+void reg_op(std::vector<C<T>>& v, const ketk::RunOp& op, uint32_t base, uint64_t fixed,
+            const unsigned char* rec) {
+  const int rpt = (int)v.size();
+  if ((fixed & op.out_mask) != op.out_mask) return;
+  if ((base & op.reg_lane) != op.reg_lane) return;
+  if (op.kind == ketk::RUN_PHASE) {
+    const C<T> ph{(T)op.m[0], (T)op.m[1]};
+    for (int j = 0; j < rpt; ++j)
+      if ((op.reg_mask >> j) & 1) v[j] = cmul(v[j], ph);
+    return;
+  }
+  if (op.kind == ketk::RUN_MATRIX) {
+    const int D = 1 << op.npos;
+    const unsigned char* mat = rec + op.mat_offset;
+    for (int g = 0; g < rpt / D; ++g) {
+      std::vector<C<T>> in(v.begin() + g * D, v.begin() + (g + 1) * D);
+      for (int r = 0; r < D; ++r) {
+        C<T> acc{T(0), T(0)};
+        for (int c = 0; c < D; ++c) {
+          double re, im;
+          std::memcpy(&re, mat + 16 * (r * D + c), 8);
+          std::memcpy(&im, mat + 16 * (r * D + c) + 8, 8);
+          acc = cadd(acc, cmul(C<T>{(T)re, (T)im}, in[c]));
+        }
+        v[g * D + r] = acc;
+      }
+    }
+    return;
+  }
+  const uint32_t pr = op.reg_part;
+  if (pr == 0 || pr >= (uint32_t)rpt) bad_record("pair partner outside the registers");
+  const C<T> m00{(T)op.m[0], (T)op.m[1]}, m01{(T)op.m[2], (T)op.m[3]}, m10{(T)op.m[4], (T)op.m[5]},
+      m11{(T)op.m[6], (T)op.m[7]};
+  for (int j = 0; j < rpt; ++j) {
+    if (!((op.reg_mask >> j) & 1)) continue;
+    const C<T> a = v[j], b = v[j ^ pr];
+    C<T> x, y;
+    if (op.kind == ketk::RUN_DENSE) {
+      x = cadd(cmul(m00, a), cmul(m01, b));
+      y = cadd(cmul(m10, a), cmul(m11, b));
+    } else if (op.kind == ketk::RUN_CH) {
+      const T r = m00.x;
+      x = {(a.x + b.x) * r, (a.y + b.y) * r};
+      y = {(a.x - b.x) * r, (a.y - b.y) * r};
+    } else {
+      x = b;
+      y = a;
+    }
+    v[j] = x;
+    v[j ^ pr] = y;
+  }
+}
+
+// Checks a run record's layouts before any use: each segment's register and
+// thread bits partition the tile bits, its slots cover the tile once, its
+// ops move only register bits (matrix bits as register bits 0..k-1), and the
+// register form of every op tests exactly the bits of its local form.
+// Returns the LDS bank-row conflicts of the segments' accesses.
+uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& ops,
+                      const std::vector<ketk::RunSeg>& segs, int tb, int row_bits) {
+  const int rpt = 1 << (tb - 8);
+  const uint32_t all = (1u << tb) - 1;
+  for (int b = 0; b < tb; ++b)
+    if (h.sbit[b] != ketk::run_slot(1u << b, row_bits)) bad_record("slot table");
+  int next = 0;
+  uint64_t conflicts = 0;
+  for (const ketk::RunSeg& sg : segs) {
+    if (sg.first_op != next || sg.n_ops < 1) bad_record("segments do not tile the ops");
+    next += sg.n_ops;
+    uint32_t tbits = 0;
+    for (int i = 0; i < 8; ++i) {
+      if (__builtin_popcount(sg.tbit[i]) != 1 || (tbits & sg.tbit[i])) bad_record("thread bits");
+      tbits |= sg.tbit[i];
+      if (sg.tslot[i] != ketk::run_slot(sg.tbit[i], row_bits)) bad_record("thread slots");
+    }
+    if ((tbits & sg.rmask) || (tbits | sg.rmask) != all ||
+        __builtin_popcount(sg.rmask) != tb - 8)
+      bad_record("layout is not a partition of the tile bits");
+    int rpos[4], nr = 0;
+    for (int r = 0; r < tb - 8; ++r) {
+      const uint32_t b = sg.roff[1u << r];
+      if (__builtin_popcount(b) != 1 || !(b & sg.rmask)) bad_record("register bits");
+      rpos[nr++] = __builtin_ctz(b);
+    }
+    for (int j = 0; j < rpt; ++j) {
+      uint32_t t = 0;
+      for (int r = 0; r < nr; ++r)
+        if ((j >> r) & 1) t |= 1u << rpos[r];
+      if (sg.roff[j] != t || sg.rslot[j] != ketk::run_slot(t, row_bits)) bad_record("register slots");
+    }
+    std::vector<int> seen(std::size_t{1} << tb, 0);
+    for (uint32_t tid = 0; tid < 256; ++tid) {
+      uint32_t base = 0, slot = 0;
+      for (int i = 0; i < 8; ++i)
+        if ((tid >> i) & 1) {
+          base |= sg.tbit[i];
+          slot ^= sg.tslot[i];
+        }
+      for (int j = 0; j < rpt; ++j) {
+        const uint32_t t = base | sg.roff[j], sl = slot ^ sg.rslot[j];
+        if (sl != ketk::run_slot(t, row_bits) || sl > all || seen[sl]++) bad_record("slots");
+      }
+    }
+    // lanes that share the LDS banks in one access: 2^row_bits consecutive
+    // lanes must reach distinct slots of a 128-byte row
+    for (int j = 0; j < rpt; ++j)
+      for (uint32_t g = 0; g < 64; g += 1u << row_bits) {
+        std::vector<int> rows(1u << row_bits, 0);
+        for (uint32_t l = g; l < g + (1u << row_bits); ++l) {
+          uint32_t slot = 0;
+          for (int i = 0; i < 8; ++i)
+            if ((l >> i) & 1) slot ^= sg.tslot[i];
+          if (rows[(slot ^ sg.rslot[j]) & ((1u << row_bits) - 1)]++) ++conflicts;
+        }
+      }
+    for (int o = sg.first_op; o < sg.first_op + sg.n_ops; ++o) {
+      const ketk::RunOp& op = ops[o];
+      if (op.kind == ketk::RUN_MATRIX) {
+        for (int b = 0; b < op.npos; ++b)
+          if (op.mpos[b] != rpos[b]) bad_record("matrix bits are not register bits 0..k-1");
+        continue;
+      }
+      auto to_local = [&](uint32_t r) {
+        uint32_t t = 0;
+        for (int b = 0; b < nr; ++b)
+          if ((r >> b) & 1) t |= 1u << rpos[b];
+        return t;
+      };
+      if ((op.partner & ~sg.rmask) != 0) bad_record("partner bits outside the registers");
+      if (to_local(op.reg_part) != op.partner || to_local(op.reg_lo) != (op.set & op.partner))
+        bad_record("register form of the partner");
+      const uint32_t cond = op.set & ~op.partner;
+      if ((op.reg_lane | to_local(op.reg_cond)) != cond || (op.reg_lane & sg.rmask))
+        bad_record("register form of the conditions");
+      uint32_t want = 0;
+      for (int j = 0; j < rpt; ++j)
+        if (((uint32_t)j & op.reg_cond) == op.reg_cond && ((uint32_t)j & op.reg_part) == op.reg_lo)
+          want |= 1u << j;
+      if (op.reg_mask != want) bad_record("register mask");
+      if (op.kind == ketk::RUN_PHASE) {
+        int n = 0;
+        for (int o2 = o; o2 < sg.first_op + sg.n_ops && ops[o2].kind == ketk::RUN_PHASE; ++o2) ++n;
+        if (op.stretch != n) bad_record("phase stretch length");
+      }
+    }
+  }
+  if (next != h.n_ops) bad_record("segments do not cover the ops");
+  return conflicts;
+}
+
+uint64_t g_bank_conflicts = 0;
+
+// k_run_tile's walk over one run record, thread by thread and segment by
+// segment, on the same LDS slots.
 template <class T>
 void run_record(C<T>* s, int n, int precision, const unsigned char* rec) {
   ketk::RunRecord h;
   std::memcpy(&h, rec, sizeof h);
   std::vector<ketk::RunOp> ops(h.n_ops);
   std::memcpy(ops.data(), rec + sizeof h, ops.size() * sizeof(ketk::RunOp));
+  std::vector<ketk::RunSeg> segs(h.n_segs);
+  std::memcpy(segs.data(), rec + sizeof h + ops.size() * sizeof(ketk::RunOp),
+              segs.size() * sizeof(ketk::RunSeg));
   const int tb = ketk::run_tile_bits(precision);
+  const int row_bits = precision == 0 ? 3 : 4;
+  const int rpt = 1 << (tb - 8);
+  g_bank_conflicts += check_record(h, ops, segs, tb, row_bits);
   std::vector<C<T>> L(std::size_t{1} << tb);
+  std::vector<C<T>> v(rpt);
   for (uint64_t tile = 0; tile < (uint64_t{1} << (n - tb)); ++tile) {
     const uint64_t fixed = deposit_outside(tile, h.pmask);
-    for (uint64_t t = 0; t < L.size(); ++t) L[t] = s[fixed | deposit(t, h.pmask)];
-    for (const ketk::RunOp& op : ops) {
-      if ((fixed & op.out_mask) != op.out_mask) continue;
-      const uint64_t items = L.size() >> op.npos;
-      if (op.kind == ketk::RUN_PHASE) {
-        phase(L.data(), items, op.npos, op.pos, op.set, op.m);
-      } else if (op.kind == ketk::RUN_MATRIX) {
-        uint64_t off[8];
-        for (int e = 0; e < (1 << op.npos); ++e) {
-          off[e] = 0;
-          for (int b = 0; b < op.npos; ++b)
-            if ((e >> b) & 1) off[e] |= uint64_t{1} << op.mpos[b];
-        }
-        matrix(L.data(), L.size(), op.npos, op.pos, off, rec + op.mat_offset);
-      } else {
-        pair(L.data(), items, op.npos, op.pos, op.set, op.partner, op.kind, op.m);
+    for (uint64_t t = 0; t < L.size(); ++t)
+      L[ketk::run_slot((uint32_t)t, row_bits)] = s[fixed | deposit(t, h.pmask)];
+    for (const ketk::RunSeg& sg : segs)
+      for (uint32_t tid = 0; tid < 256; ++tid) {
+        uint32_t base = 0, slot = 0;
+        for (int i = 0; i < 8; ++i)
+          if ((tid >> i) & 1) {
+            base |= sg.tbit[i];
+            slot ^= sg.tslot[i];
+          }
+        for (int j = 0; j < rpt; ++j) v[j] = L[slot ^ sg.rslot[j]];
+        for (int o = sg.first_op; o < sg.first_op + sg.n_ops; ++o) reg_op(v, ops[o], base, fixed, rec);
+        for (int j = 0; j < rpt; ++j) L[slot ^ sg.rslot[j]] = v[j];
       }
-    }
-    for (uint64_t t = 0; t < L.size(); ++t) s[fixed | deposit(t, h.pmask)] = L[t];
+    for (uint64_t t = 0; t < L.size(); ++t)
+      s[fixed | deposit(t, h.pmask)] = L[ketk::run_slot((uint32_t)t, row_bits)];
   }
 }
 
@@ -223,7 +392,8 @@ int compare(const std::vector<ket_gate>& g, int n, int precision, bool reference
       ++runs;
       in_runs += o.run_ops;
     }
-  std::printf("ok %zu %d %d\n", a.ops.size(), runs, in_runs);
+  std::printf("ok %zu %d %d %llu\n", a.ops.size(), runs, in_runs,
+              (unsigned long long)g_bank_conflicts);
   return 0;
 }
 

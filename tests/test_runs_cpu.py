@@ -1,7 +1,8 @@
 """Exact runs on the CPU (tests/cpp/test_runs.cpp, no GPU): circuits planned
 one pass per gate and with consecutive passes grouped into runs on LDS tiles
 are both executed on the CPU, the runs through their records exactly as
-k_run_tile walks them, and must give equal bits; for KET_FUSE_NONE against
+k_run_tile walks them (segment layouts thread by thread, ops in their
+register form, through the LDS swizzle), and must give equal bits; for KET_FUSE_NONE against
 KET_FUSE_EXACT and for ket's Fuser with runs off against on."""
 import os
 import subprocess
@@ -43,7 +44,7 @@ def test_runs_equal_per_gate_passes(name, precision, mode):
         r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), mode],
                            capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    ok, passes, runs, in_runs = r.stdout.split()
+    ok, passes, runs, in_runs, conflicts = r.stdout.split()
     assert ok == "ok" and int(runs) >= 1 and int(in_runs) >= 2
 
 
