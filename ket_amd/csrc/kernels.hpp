@@ -84,20 +84,21 @@ hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
 enum RunKind { RUN_DENSE = 0, RUN_CH = 1, RUN_SWAP = 2, RUN_PHASE = 3, RUN_MATRIX = 4 };
 struct RunOp {
   // read by k_run_tile: the op in its segment's register layout (matrix
-  // ops: matrix bit b is register bit b, so the reg_* fields stay 0)
-  int32_t kind;
-  int32_t npos;        // local positions in pos (matrix: its width k)
+  // ops: matrix bit b is register bit b, so the reg_* fields stay 0). The
+  // first 32 bytes are all a phase op needs, so they load as one.
   uint64_t out_mask;   // bits outside P that must all be set (outside controls)
   uint32_t reg_lane;   // tile bits held by thread bits that must be set
-  uint32_t reg_cond;   // register-index bits that must be set
-  uint32_t reg_part;   // register-index bits a pair's two amplitudes differ in
-  uint32_t reg_lo;     // their values in the pair's lower amplitude
   uint32_t reg_mask;   // registers j the op applies to: phases, j holding
                        // reg_cond; pairs, the lower amplitudes (j's reg_part
                        // bits read reg_lo) holding reg_cond
+  double m[8];         // 2x2 row-major (re, im); CH: m[0]; phase: m[0], m[1]
+  int32_t kind;
+  int32_t npos;        // local positions in pos (matrix: its width k)
+  uint32_t reg_cond;   // register-index bits that must be set
+  uint32_t reg_part;   // register-index bits a pair's two amplitudes differ in
+  uint32_t reg_lo;     // their values in the pair's lower amplitude
   uint32_t stretch;    // phase ops: consecutive phase ops of the segment from
                        // this one on (this one included)
-  double m[8];         // 2x2 row-major (re, im); CH: m[0]; phase: m[0], m[1]
   uint32_t mat_offset; // matrix ops: bytes from the record start
   int32_t pad;
   // the op's local form (planner, CPU interpreter)

@@ -1036,18 +1036,50 @@ __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double
 // register's update is a branch around it alone, so the registers stay in
 // place from op to op. base = the tile-index bits of the thread's lanes,
 // fixed = the tile's bits outside P.
+// A wave-uniform double through readfirstlane (which takes and returns a
+// 32-bit int: each half is taken as unsigned before it is widened).
+__device__ __forceinline__ double uniform_f64(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 template <class T, int RPT>
 __device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunOp* opp, int count,
                                            uint32_t base, uint64_t fixed) {
-  for (int q = 0; q < count; ++q, ++opp) {
-    const uint64_t out = opp->out_mask;
-    const uint32_t lane = opp->reg_lane, mask = opp->reg_mask;
-    const uint32_t rm = (fixed & out) == out ? mask : 0u;
-    const ketd::cplx<T> ph = ketd::mk<T>((T)opp->m[0], (T)opp->m[1]);
-    if ((base & lane) == lane) {  // per lane
+  // Groups of G ops: the fields of a group load together (indices clamped to
+  // the stretch), so one wait for the scalar loads covers G ops.
+  constexpr int G = 4;
+  struct Head {  // the first 32 bytes of a RunOp
+    uint64_t out;
+    uint32_t lane, mask;
+    double re, im;
+  };
+  for (int q0 = 0; q0 < count; q0 += G) {
+    uint32_t lane[G], rm[G];
+    ketd::cplx<T> ph[G];
+    Head hd[G];
 #pragma unroll
-      for (int j = 0; j < RPT; ++j)
-        if ((rm >> j) & 1) v[j] = ketd::cmul<T>(v[j], ph);  // uniform
+    for (int i = 0; i < G; ++i)
+      hd[i] = *reinterpret_cast<const Head*>(opp + (q0 + i < count ? q0 + i : count - 1));
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      // readfirstlane is convergent, so these values (and their loads) stay
+      // here rather than sinking into the op's per-lane branch
+      lane[i] = (uint32_t)__builtin_amdgcn_readfirstlane(hd[i].lane);
+      rm[i] = (uint32_t)__builtin_amdgcn_readfirstlane((fixed & hd[i].out) == hd[i].out ? hd[i].mask
+                                                                                         : 0u);
+      ph[i] = ketd::mk<T>((T)uniform_f64(hd[i].re), (T)uniform_f64(hd[i].im));
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      if (q0 + i >= count) break;  // uniform
+      if ((base & lane[i]) == lane[i]) {  // per lane
+#pragma unroll
+        for (int j = 0; j < RPT; ++j)
+          if ((rm[i] >> j) & 1) v[j] = ketd::cmul<T>(v[j], ph[i]);  // uniform
+      }
     }
   }
 }
