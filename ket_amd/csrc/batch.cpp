@@ -389,11 +389,16 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
       L.pool_off = off;
       L.rec_stride = rec;
       launches.push_back(L);
-      key += std::to_string(L.k) + "/" + std::to_string(L.n_terms) + ",";
+      key += std::to_string(L.k) + "/" + std::to_string(L.n_terms) + "/" +
+             std::to_string(L.lowest) + ",";
       st.alg_bytes += 2.0 * B * (double)(uint64_t{1} << n) * inst.size();
     }
     if (!uniform) {
-      for (std::size_t x = 0; x < inst.size(); ++x)
+      // Each launch bakes in its instance's arena offset: the key marks
+      // where every instance's passes start, so batches that split their
+      // blocks differently among instances never share a captured graph.
+      for (std::size_t x = 0; x < inst.size(); ++x) {
+        key += "|" + std::to_string(plan_of(x).ops.size()) + ":";
         for (const ketplan::Op& o : plan_of(x).ops) {
           const auto* h =
               reinterpret_cast<const ketk::BlockRecord*>(plan_of(x).pool.data() + o.pool_offset);
@@ -411,9 +416,11 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
           L.arena_off = base + (x << n);
           L.pool_off = off;
           launches.push_back(L);
-          key += "s" + std::to_string(o.k) + "/" + std::to_string(o.n_terms) + ",";
+          key += "s" + std::to_string(o.k) + "/" + std::to_string(o.n_terms) + "/" +
+                 std::to_string(L.lowest) + ",";
           st.alg_bytes += 2.0 * B * (double)(uint64_t{1} << n);
         }
+      }
     }
     // Observable: pad every instance to the group's largest term count.
     std::size_t nt = 0;

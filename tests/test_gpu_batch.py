@@ -62,6 +62,39 @@ def test_non_uniform_structures_and_all_gates():
             assert np.abs(vals - want).max() <= 1e-10, k
 
 
+def blocks_on(n, groups, seed):
+    """U gates on each listed group of qubits in turn: with block width 4 and
+    every group emitted as a block, one block per group."""
+    rng = np.random.default_rng(seed)
+    c = W.Circuit(n)
+    for g in groups:
+        for q in g:
+            c.u(q, *rng.uniform(0, 2 * np.pi, 3))
+    return c
+
+
+def test_graph_replay_follows_instance_boundaries():
+    """Two same-width instances whose passes split 3 + 2 and then 2 + 3: the
+    launch counts, pool size and per-pass shapes agree, but each launch
+    addresses its own instance's state, so the second sweep must not replay
+    the first sweep's graph."""
+    n = 14
+    three = [range(0, 4), range(4, 8), range(8, 12)]
+    two = [range(0, 4), range(6, 10)]
+    obs = [zz_obs(n), zz_obs(n)]
+    with N.Batch([n, n], N.C128) as b:
+        first = [blocks_on(n, three, 1), blocks_on(n, two, 2)]
+        v1, s1 = b.run(first, obs, block_k=4, use_graph=True)
+        second = [blocks_on(n, two, 3), blocks_on(n, three, 4)]
+        v2, s2 = b.run(second, obs, block_k=4, use_graph=True)
+        v3, s3 = b.run(second, obs, block_k=4, use_graph=True)
+        assert s1["instances_hbm"] == 2 and s1["launches"] == s2["launches"]
+        assert s2["graph_replayed"] == 0 and s3["graph_replayed"] == 1
+        for vals, circs in ((v1, first), (v2, second), (v3, second)):
+            want = np.array([oracle_value(c, o) for c, o in zip(circs, obs)])
+            assert np.abs(vals - want).max() <= 1e-10
+
+
 def test_batch_rejects_bad_input():
     with pytest.raises(ValueError):
         N.Batch([3, 31])
