@@ -64,7 +64,10 @@ class GpuStateVectorSimulation final : public Simulation {
   std::vector<int> sample(std::mt19937& rng) const override {
     const double u = detail::draw_uniform(rng);
     std::uint64_t outcome = 0;
-    detail::check(ket_sv_sample(dev_->get(), &u, 1, &outcome));
+    {
+      const auto lk = dev_->lock();
+      detail::check(ket_sv_sample(dev_->get(), &u, 1, &outcome));
+    }
     return creg(outcome);
   }
 
@@ -76,7 +79,7 @@ class GpuStateVectorSimulation final : public Simulation {
     return r;
   }
 
-  ket_sv* sv() const { return dev_->get(); }
+  const detail::DeviceState& dev() const { return *dev_; }
 
  private:
   std::shared_ptr<detail::DeviceState> dev_;
@@ -112,12 +115,17 @@ double GpuStateVectorSimulation::expval(const std::string& pauli) const {
   int n_y;
   pauli_masks(pauli, dev_->n_qubits(), &flip, &sign, &n_y);
   double v = 0.0;
+  const auto lk = dev_->lock();
   detail::check(ket_sv_expval_pauli(dev_->get(), flip, sign, n_y, &v));
   return v;
 }
 
 // One-entry cache of the last evolution, keyed by the flattened records and
-// the options they were evolved with.
+// the options they were evolved with: repeated ket::sample(circuit, rng)
+// calls (each a new Simulation, src/measurement.cpp:28-31) reuse one
+// evolution. Simulations share its state, and their calls on it are
+// serialized by the state's lock. The entry is dropped before any new device
+// state is allocated, so it never holds memory a new state needs.
 struct EvolutionCache {
   std::mutex mu;
   std::vector<ket_gate> key;
@@ -141,7 +149,7 @@ std::shared_ptr<detail::DeviceState> evolved(const Circuit& c) {
         g_cache.opt.block_k == opt.block_k)
       return g_cache.dev;
   }
-  auto dev = detail::evolve(c, nullptr);
+  auto dev = detail::evolve(c, nullptr);  // its DeviceState dropped the old entry
   std::lock_guard<std::mutex> lk(g_cache.mu);
   g_cache.key = flat.gates;
   g_cache.n = c.n_qubits();
@@ -151,6 +159,18 @@ std::shared_ptr<detail::DeviceState> evolved(const Circuit& c) {
 }
 
 }  // namespace
+
+namespace detail {
+void release_cached_evolution() {
+  std::shared_ptr<DeviceState> old;
+  {
+    std::lock_guard<std::mutex> lk(g_cache.mu);
+    old.swap(g_cache.dev);
+    g_cache.key.clear();
+  }
+  // `old` is released here, outside the lock (freeing a state syncs the GPU)
+}
+}  // namespace detail
 
 std::unique_ptr<Simulation> simulate(const Circuit& circuit, Method method) {
   if (method == Method::Auto) method = chosen_method(circuit);
@@ -189,7 +209,8 @@ std::vector<std::uint64_t> sample_indices(const Simulation& sim, std::mt19937& r
   std::vector<double> u(shots);
   for (double& x : u) x = detail::draw_uniform(rng);
   std::vector<std::uint64_t> out(shots);
-  detail::check(ket_sv_sample(g->sv(), u.data(), shots, out.data()));
+  const auto lk = g->dev().lock();
+  detail::check(ket_sv_sample(g->dev().get(), u.data(), shots, out.data()));
   return out;
 }
 

@@ -24,6 +24,7 @@ void check(int status) {
 
 DeviceState::DeviceState(std::size_t n_qubits, int precision)
     : n_(n_qubits), precision_(precision) {
+  release_cached_evolution();
   check(ket_sv_create(static_cast<int>(n_qubits), precision, &sv_));
 }
 

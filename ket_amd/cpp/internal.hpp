@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,9 +19,15 @@ namespace ket::detail {
 // Throws the exception type ket uses for a C ABI status.
 void check(int status);
 
-// A device state (RAII over ket_sv).
+// A device state (RAII over ket_sv). A ket_sv's calls must not overlap (its
+// staging buffers and sampler tables are per state), so callers that can
+// share one state across threads (Simulations of one cached evolution) hold
+// lock() around their calls.
 class DeviceState {
  public:
+  // Drops the process's cached evolution first (release_cached_evolution),
+  // so that its memory is free for this state unless a Simulation still
+  // holds it.
   DeviceState(std::size_t n_qubits, int precision);
   ~DeviceState();
   DeviceState(const DeviceState&) = delete;
@@ -30,12 +37,17 @@ class DeviceState {
   int precision() const { return precision_; }
   State to_host() const;
   void from_host(const State& s);
+  std::unique_lock<std::mutex> lock() const { return std::unique_lock<std::mutex>(mu_); }
 
  private:
   ket_sv* sv_ = nullptr;
   std::size_t n_;
   int precision_;
+  mutable std::mutex mu_;
 };
+
+// Releases the one-entry evolution cache behind simulate() (backend.cpp).
+void release_cached_evolution();
 
 // Flattened gate records of a circuit: composites expand with their wire map
 // between BARRIER records (each composite runs its own Fuser in ket,

@@ -4,12 +4,14 @@
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include <ket/amd.hpp>
 #include <ket/backends/backend.hpp>
 #include <ket/circuit.hpp>
 #include <ket/measurement.hpp>
+#include <ket/observable.hpp>
 #include <ket/simulator.hpp>
 
 namespace {
@@ -96,6 +98,44 @@ int main() {
       threw = true;
     }
     check(threw, "invalid block width throws std::invalid_argument");
+  }
+  // Threads sharing one cached evolution: ket::sample(circuit, rng) from 8
+  // threads at once (each call a new Simulation over the same device state)
+  // gives every thread exactly the shots it gets alone.
+  {
+    constexpr int kThreads = 8, kShots = 200;
+    std::vector<std::vector<std::vector<int>>> alone(kThreads), together(kThreads);
+    for (int t = 0; t < kThreads; ++t) {
+      std::mt19937 rng{static_cast<unsigned>(100 + t)};
+      for (int i = 0; i < kShots; ++i) alone[t].push_back(ket::sample(c, rng));
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < kThreads; ++t)
+      pool.emplace_back([&, t] {
+        std::mt19937 rng{static_cast<unsigned>(100 + t)};
+        for (int i = 0; i < kShots; ++i) together[t].push_back(ket::sample(c, rng));
+      });
+    for (auto& th : pool) th.join();
+    check(alone == together, "concurrent ket::sample calls equal the same calls made alone");
+  }
+  // expval(State, PauliSum): one upload, diagonal terms in one pass; equal to
+  // the term-by-term sum ket computes to rounding.
+  {
+    const ket::State s = ket::run(c);
+    const ket::PauliSum h = {{0.5, "ZZIIIIIIIIIZ"}, {-1.25, "IXIYIIZIIIII"},
+                             {0.75, "IIIIIIIIIIZZ"}, {2.0, "ZIZIZIZIZIZI"},
+                             {-0.3, "XXXXXXXXXXXX"}, {1.0, "IIIIIIIIIIII"}};
+    double by_term = 0.0;
+    for (const auto& [cf, p] : h) by_term += cf * ket::expval(s, p);
+    check(std::fabs(ket::expval(s, h) - by_term) <= 1e-12, "expval(State, PauliSum) sums its terms");
+    check(ket::expval(s, ket::PauliSum{}) == 0.0, "empty PauliSum gives 0");
+    bool threw = false;
+    try {
+      (void)ket::expval(s, ket::PauliSum{{1.0, "ZZ"}});
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    check(threw, "PauliSum term of the wrong length throws std::invalid_argument");
   }
   std::printf("%d failures\n", failures);
   return failures;

@@ -266,7 +266,7 @@ uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& 
       if (op.kind == ketk::RUN_PHASE) {
         int n = 0;
         for (int o2 = o; o2 < sg.first_op + sg.n_ops && ops[o2].kind == ketk::RUN_PHASE; ++o2) ++n;
-        if (op.stretch != n) bad_record("phase stretch length");
+        if (op.stretch != (uint32_t)n) bad_record("phase stretch length");
       }
     }
   }
