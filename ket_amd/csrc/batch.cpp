@@ -464,8 +464,7 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     b->pool_cap = 0;
     KET_HIP_TRY(hipMalloc(&b->pool, pool.size()), "hipMalloc(batch pool)");
     b->pool_cap = pool.size();
-    key += "!";  // a new pool address invalidates a captured graph
-    b->graph_key.clear();
+    b->graph_key.clear();  // a new pool address invalidates a captured graph
   }
   if (b->pool_host_cap < pool.size()) {
     if (b->pool_host) (void)hipHostFree(b->pool_host);

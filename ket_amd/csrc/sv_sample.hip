@@ -250,7 +250,6 @@ __global__ __launch_bounds__(256) void k_exact_prefix(
   if (threadIdx.x == 0) exact[n_chunks] = c_sh;
 }
 
-// One wave per draw.
 // ket's running sum at one index per thread: the exact chunk start, then
 // the chunk's probabilities up to and including idx in index order, the
 // additions measure() makes (src/measurement.cpp:15-20).
@@ -267,6 +266,7 @@ __global__ __launch_bounds__(256) void k_prob_prefix(const ketd::cplx<T>* s, int
   out[q] = c;
 }
 
+// One wave per draw.
 template <class T>
 __global__ __launch_bounds__(256) void k_draw(
     const ketd::cplx<T>* s, uint64_t dim, int chunk_log, uint64_t n_chunks,

@@ -44,6 +44,22 @@ def test_hea_sweep_matches_oracle(precision, tol):
         assert np.abs(v3 - want).max() <= tol
 
 
+@pytest.mark.parametrize("precision,tol", [(N.C128, 1e-10), (N.C64, 2e-4)])
+def test_hea_widths_17_to_22_in_the_arena(precision, tol):
+    """The top of the sweep's width range (config 4 draws widths 10..22):
+    instances of 17..22 qubits in the HBM arena, with graph capture and
+    replay, against the oracle's run + expval."""
+    ns = [17, 18, 19, 20, 21, 22, 17, 22]
+    circs = [W.hea(n, 8, seed=300 + i) for i, n in enumerate(ns)]
+    obs = [zz_obs(n) for n in ns]
+    want = np.array([oracle_value(c, o) for c, o in zip(circs, obs)])
+    with N.Batch(ns, precision) as b:
+        v1, s1 = b.run(circs, obs, block_k=0, use_graph=True)
+        v2, s2 = b.run(circs, obs, block_k=0, use_graph=True)
+    assert s1["instances_hbm"] == len(ns) and s2["graph_replayed"] == 1
+    assert np.abs(v1 - want).max() <= tol and np.abs(v2 - want).max() <= tol
+
+
 def test_non_uniform_structures_and_all_gates():
     rng = np.random.default_rng(3)
     ns = [int(x) for x in rng.integers(3, 16, 24)]
