@@ -138,7 +138,11 @@ int ket_sv_apply_matrix(ket_sv* sv, int k, const int* qubits,
 
 /* ---- circuits: apply_circuit + Fuser (src/simulator.cpp:400-517) ---- */
 /* block_k: largest fused block for KET_FUSE_BLOCKS/LEARNED (2..6; 0: 4 for
- * complex128, 5 for complex64). stats may be NULL. */
+ * complex128, 5 for complex64). stats may be NULL. The state keeps its last
+ * plan: the same records with the same fusion and block_k (and no launch
+ * tuning change since) are not planned again, nor their tables uploaded
+ * again while the state's device pool still holds them; KET_FUSE_LEARNED
+ * plans afresh every call. */
 int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
                          int fusion, int block_k, ket_run_stats* stats);
 

@@ -16,22 +16,22 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstring>
-#include <deque>
 #include <map>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
 #include "ket_amd.h"
+#include "batch_plan.hpp"
 #include "plan.hpp"
 #include "status.hpp"
 
 using ketabi::fail;
 using ketabi::hip_fail;
+using ketbatch::align;
+using ketbatch::PoolBuilder;
 
 struct ket_batch {
   int precision = 0;
@@ -73,109 +73,6 @@ struct Launch {
   int lowest = -1;             // Block, BlockSingle: the block's lowest qubit
 };
 
-std::size_t align(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
-
-
-// fn(x) for x in [0, count) on up to 16 host threads.
-template <class F>
-void parallel_for(std::size_t count, F fn) {
-  unsigned T = std::thread::hardware_concurrency();
-  T = std::max(1u, std::min<unsigned>(T, 16));
-  T = (unsigned)std::min<std::size_t>(T, count);
-  std::atomic<std::size_t> next{0};
-  auto work = [&]() {
-    for (std::size_t x = next++; x < count; x = next++) fn(x);
-  };
-  std::vector<std::thread> pool_threads;
-  for (unsigned t = 1; t < T; ++t) pool_threads.emplace_back(work);
-  work();
-  for (std::thread& t : pool_threads) t.join();
-}
-
-// The upload pool of one sweep, laid out first and written once: small
-// tables are kept here until the upload; block records are copied straight
-// from the instances' plans into the pinned staging buffer, in parallel.
-class PoolBuilder {
- public:
-  std::size_t size() const { return size_; }
-  std::size_t reserve(std::size_t bytes) {
-    const std::size_t off = align(size_, 256);
-    size_ = off + bytes;
-    return off;
-  }
-  template <class T>
-  std::size_t put(const std::vector<T>& v) {
-    const std::size_t bytes = v.size() * sizeof(T);
-    const std::size_t off = reserve(bytes);
-    if (bytes) {
-      const auto* p = reinterpret_cast<const unsigned char*>(v.data());
-      owned_.emplace_back(p, p + bytes);
-      jobs_.push_back({off, owned_.back().data(), bytes});
-    }
-    return off;
-  }
-  // `src` must stay valid until write_to().
-  void copy_later(std::size_t dst, const void* src, std::size_t bytes) {
-    jobs_.push_back({dst, src, bytes});
-  }
-  void write_to(unsigned char* dst) const {
-    const std::size_t per = 64, chunks = (jobs_.size() + per - 1) / per;
-    parallel_for(chunks, [&](std::size_t c) {
-      for (std::size_t j = c * per; j < std::min(jobs_.size(), (c + 1) * per); ++j)
-        std::memcpy(dst + jobs_[j].dst, jobs_[j].src, jobs_[j].n);
-    });
-  }
-
- private:
-  struct Job {
-    std::size_t dst;
-    const void* src;
-    std::size_t n;
-  };
-  std::size_t size_ = 0;
-  std::vector<Job> jobs_;
-  std::deque<std::vector<unsigned char>> owned_;
-};
-
-ketk::LdsOp lds_op(const ketplan::Op& o) {
-  ketk::LdsOp r{};
-  uint64_t fixed = 0;
-  for (int i = 0; i < o.spec.npos; ++i) fixed |= uint64_t{1} << o.spec.pos[i];
-  r.fixed = fixed;
-  r.set = o.spec.set_mask;
-  r.partner = o.spec.partner;
-  if (o.kind == ketplan::Kind::Phase) {
-    r.kind = 3;
-    r.m[0] = o.m[0];
-    r.m[1] = o.m[1];
-  } else {
-    r.kind = o.pair_op == ketk::OP_DENSE ? 0 : o.pair_op == ketk::OP_CH ? 1 : 2;
-    std::memcpy(r.m, o.m, sizeof r.m);
-  }
-  return r;
-}
-
-// Blocks-only plan of one instance.
-int plan_blocks_only(const ket_gate* g, std::size_t ng, int n, int precision, int block_k,
-                     ketplan::Plan* plan, std::string* err) {
-  *plan = ketplan::Plan{};
-  std::vector<std::size_t> run;
-  for (std::size_t i = 0; i <= ng; ++i) {
-    if (i < ng) {
-      if (int st = ketplan::validate(g[i], n, err)) return st;
-    }
-    const bool unitary = i < ng && g[i].type >= KET_H && g[i].type <= KET_CSWAP;
-    if (unitary) {
-      ++plan->gates;
-      run.push_back(i);
-      continue;
-    }
-    if (!run.empty())
-      ketplan::pack_blocks(g, run, n, precision, std::min(block_k, n), plan, {}, true);
-    run.clear();
-  }
-  return KET_OK;
-}
 
 bool same_structure(const ketplan::Plan& a, const ketplan::Plan& b) {
   if (a.ops.size() != b.ops.size()) return false;
@@ -281,29 +178,14 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
   // LDS ops of LDS-resident instances, the blocks-only plans of the others ----
   std::vector<char> in_lds(ni, 0);
   for (std::size_t i : b->lds_inst) in_lds[i] = 1;
-  std::vector<std::vector<ketk::LdsOp>> lds_ops(ni);
-  std::vector<ketplan::Plan> inst_plans(ni);
-  std::vector<int> status(ni, KET_OK);
-  std::vector<std::string> errs(ni);
-  parallel_for(ni, [&](std::size_t i) {
-    const int n = b->n[i];
-    const ket_gate* g = gates + gate_off[i];
-    const std::size_t ng = gate_off[i + 1] - gate_off[i];
-    if (!in_lds[i]) {
-      status[i] = plan_blocks_only(g, ng, n, P, block_k, &inst_plans[i], &errs[i]);
-      return;
-    }
-    for (std::size_t j = 0; j < ng; ++j) {
-      if (int s2 = ketplan::validate(g[j], n, &errs[i])) {
-        status[i] = s2;
-        return;
-      }
-      if (g[j].type < KET_H || g[j].type > KET_CSWAP) continue;
-      lds_ops[i].push_back(lds_op(ketplan::op_for_gate(g[j], n, P)));
-    }
-  });
-  for (std::size_t i = 0; i < ni; ++i)
-    if (status[i] != KET_OK) return fail(status[i], errs[i]);
+  std::vector<std::vector<ketk::LdsOp>> lds_ops;
+  std::vector<ketplan::Plan> inst_plans;
+  {
+    std::string err;
+    if (int s2 = ketbatch::plan_instances(b->n, in_lds, gates, gate_off, P, block_k, &lds_ops,
+                                          &inst_plans, &err))
+      return fail(s2, err);
+  }
   const auto t_pack = std::chrono::steady_clock::now();
 
   // ---- LDS-resident instances, one launch per width ----

@@ -57,6 +57,18 @@ struct ket_sv {
   size_t io_cap = 0;
   hipEvent_t ev[8] = {};
   ket_run_stats last{};
+  uint64_t pool_uploads = 0;  // uploads into `pool` so far
+  // The last circuit's plan, reused when the same records arrive again with
+  // the same fusion mode, block width and launch tuning (a repeated step
+  // neither replans nor, while the device pool still holds it, re-uploads).
+  struct {
+    bool valid = false;
+    std::vector<ket_gate> gates;
+    int fusion = 0, block_k = 0;
+    uint64_t tuning = 0;
+    uint64_t upload = 0;  // pool_uploads right after its pool was uploaded
+    ketplan::Plan plan;
+  } cached;
 };
 
 namespace {
@@ -140,9 +152,12 @@ int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
   return KET_OK;
 }
 
-int run_plan(ket_sv* sv, const ketplan::Plan& plan) {
+// Launches a plan; pool_current: the device pool already holds plan.pool.
+int run_plan(ket_sv* sv, const ketplan::Plan& plan, bool pool_current = false) {
   const unsigned char* pool_dev = nullptr;
-  if (!plan.pool.empty()) {
+  if (!plan.pool.empty() && pool_current) {
+    pool_dev = static_cast<const unsigned char*>(sv->pool);
+  } else if (!plan.pool.empty()) {
     // The previous upload must have left the pinned staging buffer before it
     // is rewritten; kernels of earlier plans that still read the device pool
     // are ordered before this copy on the same stream.
@@ -163,6 +178,7 @@ int run_plan(ket_sv* sv, const ketplan::Plan& plan) {
             "hipMemcpyAsync(pool)");
     HIP_TRY(hipEventRecord(sv->pool_ev, sv->stream), "hipEventRecord(pool)");
     pool_dev = static_cast<const unsigned char*>(sv->pool);
+    ++sv->pool_uploads;
   }
   for (const ketplan::Op& o : plan.ops) {
     int st = launch_op(sv, o, pool_dev);
@@ -491,6 +507,20 @@ int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int 
   if (int st = check_sv(sv)) return st;
   if (n_gates && !gates) return fail(KET_ERR_ARG, "ket_amd: null gate list");
   const auto t0 = std::chrono::steady_clock::now();
+  auto& c = sv->cached;
+  // The learned planner's model can change between calls: its plans are
+  // made afresh.
+  const bool cacheable = fusion != KET_FUSE_LEARNED;
+  const uint64_t tuning = ketk::tuning_generation();
+  if (cacheable && c.valid && c.fusion == fusion && c.block_k == block_k && c.tuning == tuning &&
+      c.gates.size() == n_gates &&
+      (n_gates == 0 || std::memcmp(c.gates.data(), gates, n_gates * sizeof(ket_gate)) == 0)) {
+    const double plan_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_stats(sv, c.plan, plan_s, &sv->last);
+    if (stats) *stats = sv->last;
+    return run_plan(sv, c.plan, c.upload == sv->pool_uploads);
+  }
   ketplan::Plan plan;
   std::string err;
   if (int st = ketplan::plan_circuit(gates, n_gates, sv->n, sv->precision, fusion, block_k,
@@ -500,7 +530,18 @@ int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int 
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   fill_stats(sv, plan, plan_s, &sv->last);
   if (stats) *stats = sv->last;
-  return run_plan(sv, plan);
+  c.valid = false;
+  const int st = run_plan(sv, plan);
+  if (st == KET_OK && cacheable) {
+    c.gates.assign(gates, gates + n_gates);
+    c.fusion = fusion;
+    c.block_k = block_k;
+    c.tuning = tuning;
+    c.upload = sv->pool_uploads;
+    c.plan = std::move(plan);
+    c.valid = true;
+  }
+  return st;
 }
 
 int ket_sv_profile_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int fusion,

@@ -44,6 +44,8 @@ struct Tuning {
 // Per precision (0 = complex128, 1 = complex64).
 void set_tuning(int precision, const Tuning& t);
 Tuning get_tuning(int precision);
+// Changes with every set_tuning (plans made under other tuning are stale).
+uint64_t tuning_generation();
 hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes,
                        int nt, int blocks);
 

@@ -257,18 +257,6 @@ cplx<T> to_c(const double* m, int i) {
   return r;
 }
 
-// Defaults, per precision: the autotuner's picks on MI355X
-// (ket_amd/tuning.json). Gate passes: one pair per thread (complex128) or two
-// (complex64) over a full grid; passes whose condition bits leave runs under
-// 64 bytes widened to whole lines, and nontemporal access when every 128-byte
-// line a pass writes is written whole (mode 3); the lane-pair exchange form
-// for partners fewer than 2^16 (complex128) or 2^4 (complex64) amplitudes
-// away.
-// Fused blocks: 32 workgroups per CU; complex64 blocks always through LDS
-// tiles, complex128 ones when their lowest qubit is 0..2 (always-tiles
-// measured within 0.1%), with nontemporal tile transfers. Exact runs keep
-// 3 (complex128) / 4 (complex64) low bits in every tile.
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4}};
 
 // True when every 128-byte line the pass writes is written whole: no
 // condition bit (control, phase bit, the bits of a SWAP) below the line's
@@ -316,8 +304,8 @@ inline unsigned tuned_grid(const Tuning& t, uint64_t items, int per_thread) {
 }
 
 template <class T>
-const Tuning& tuning_of() {
-  return g_tuning[sizeof(T) == 8 ? 0 : 1];
+Tuning tuning_of() {
+  return get_tuning(sizeof(T) == 8 ? 0 : 1);
 }
 
 template <class T, int OP, int ITEMS, int NT>
@@ -428,8 +416,6 @@ hipError_t launch_phase(hipStream_t st, int precision, void* s,
              : phase_impl<float>(st, s, sp, ketd::mk<float>((float)re, (float)im));
 }
 
-void set_tuning(int precision, const Tuning& t) { g_tuning[precision == 0 ? 0 : 1] = t; }
-Tuning get_tuning(int precision) { return g_tuning[precision == 0 ? 0 : 1]; }
 
 hipError_t launch_copy(hipStream_t st, const void* a, void* b, uint64_t bytes, int nt,
                        int blocks) {

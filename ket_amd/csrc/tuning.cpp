@@ -1,0 +1,34 @@
+// Launch shape and fusion width of the kernels, per precision (process-wide
+// host state; ket_set_tuning_for in the C ABI changes it).
+#include <atomic>
+
+#include "kernels.hpp"
+
+namespace ketk {
+namespace {
+
+// Defaults, per precision: the autotuner's picks on MI355X
+// (ket_amd/tuning.json). Gate passes: one pair per thread (complex128) or two
+// (complex64) over a full grid; passes whose condition bits leave runs under
+// 64 bytes widened to whole lines, and nontemporal access when every 128-byte
+// line a pass writes is written whole (mode 3); the lane-pair exchange form
+// for partners fewer than 2^16 (complex128) or 2^4 (complex64) amplitudes
+// away.
+// Fused blocks: 32 workgroups per CU; complex64 blocks always through LDS
+// tiles, complex128 ones when their lowest qubit is 0..2 (always-tiles
+// measured within 0.1%), with nontemporal tile transfers. Exact runs keep
+// 3 (complex128) / 4 (complex64) low bits in every tile.
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4}};
+
+std::atomic<uint64_t> g_generation{1};
+
+}  // namespace
+
+void set_tuning(int precision, const Tuning& t) {
+  g_tuning[precision == 0 ? 0 : 1] = t;
+  ++g_generation;
+}
+Tuning get_tuning(int precision) { return g_tuning[precision == 0 ? 0 : 1]; }
+uint64_t tuning_generation() { return g_generation.load(); }
+
+}  // namespace ketk
