@@ -414,53 +414,86 @@ def bench_brick20(args):
 
 
 def bench_planner(args):
-    """Config 5: the learned fusion planner. Stage `train`: time windows of
-    5000 generated circuits (28-32 qubits, depth 40, 13-gate set) on the GPU
-    as per-gate passes and forced blocks, train the cost model on the GPU and
-    write it to --model-out. Stage `eval`: 200 held-out
-    circuits of the same distribution run whole with the greedy packer and
-    with the learned planner (complex64); wall time of each."""
+    """Config 5: the learned fusion planner, in stages that can run in
+    separate processes (--shard i --shards S splits the circuits; each
+    stage's files go to --data-dir):
+      collect: time windows of the 5000 training circuits (28-32 qubits,
+               depth 40, 13-gate set) in every pass form -> data_<i>.npz;
+      train:   train the cost model on the GPU on every data_*.npz and
+               write it to --model-out;
+      eval:    200 held-out circuits of the same distribution, whole, in
+               every form (greedy blocks, ket's Fuser as exact runs, the
+               run packer, learned), complex64 -> eval_<i>.json;
+      report:  merge the eval_*.json into the result line;
+      all:     collect, train, eval and report in this process."""
     from ket_amd import costmodel as CM
     out = {"metric": METRIC}
+    ddir = os.path.abspath(args.data_dir)
+    os.makedirs(ddir, exist_ok=True)
+    shard = lambda xs: xs[args.shard::args.shards]
+    stages = ("collect", "train", "eval", "report") if args.stage == "all" else (args.stage,)
     states = CM.States(N.C64)
     mlp = None
     try:
-        if args.stage in ("train", "all"):
-            circs = CM.config5_circuits(args.train_circuits, seed=1234)
-            log(f"[bench] planner: timing windows of {len(circs)} training circuits")
+        if "collect" in stages:
+            circs = shard(CM.config5_circuits(args.train_circuits, seed=1234))
+            log(f"[bench] planner: timing windows of {len(circs)} training circuits "
+                f"(shard {args.shard}/{args.shards})")
             t0 = time.perf_counter()
-            X, y = CM.collect(circs, states, window_len=8, seed=1, log=log)
-            t_collect = time.perf_counter() - t0
-            log(f"[bench] {len(y)} timed passes in {t_collect:.0f} s; training")
+            X, y = CM.collect(circs, states, seed=1 + args.shard, log=log)
+            np.savez(os.path.join(ddir, f"data_{args.shard}.npz"), X=X, y=y,
+                     circuits=len(circs), seconds=time.perf_counter() - t0)
+            log(f"[bench] {len(y)} timed passes in {time.perf_counter() - t0:.0f} s")
+            out["collect"] = {"circuits": len(circs), "passes": int(len(y)),
+                              "collect_s": time.perf_counter() - t0}
+        if "train" in stages:
+            files = sorted(f for f in os.listdir(ddir) if f.startswith("data_"))
+            parts = [np.load(os.path.join(ddir, f), allow_pickle=False) for f in files]
+            X = np.concatenate([p["X"] for p in parts])
+            y = np.concatenate([p["y"] for p in parts])
+            n_circ = int(sum(int(p["circuits"]) for p in parts))
+            log(f"[bench] planner: training on {len(y)} passes of {n_circ} circuits")
             t0 = time.perf_counter()
             mlp, hist = CM.train(X, y, epochs=60, log=log)
             t_train = time.perf_counter() - t0
             dest = os.path.abspath(args.model_out)
             os.makedirs(os.path.dirname(dest), exist_ok=True)
-            CM.save(mlp, dest,
-                    train_passes=len(y), final_loss=hist[-1], epochs=len(hist))
-            out["train"] = {"circuits": len(circs), "passes": int(len(y)),
-                            "collect_s": t_collect, "train_s": t_train,
-                            "loss_first": hist[0], "loss_final": hist[-1]}
-        if args.stage in ("eval", "all"):
-            if mlp is None:  # stage eval: the shipped model
+            CM.save(mlp, dest, train_passes=len(y), train_circuits=n_circ,
+                    final_loss=hist[-1], epochs=len(hist))
+            out["train"] = {"circuits": n_circ, "passes": int(len(y)),
+                            "collect_s": float(sum(float(p["seconds"]) for p in parts)),
+                            "train_s": t_train, "loss_first": hist[0], "loss_final": hist[-1]}
+        if "eval" in stages:
+            if mlp is None:  # the shipped model
                 mlp = CM.load()
             if mlp is None:
                 raise SystemExit("no ket_amd/cost_model.npz: run --stage train first")
-            held = CM.config5_circuits(args.eval_circuits, seed=777)
-            log(f"[bench] planner: evaluating on {len(held)} held-out circuits")
-            res = CM.evaluate(mlp, held, states, log=log, ref_subset=args.ref_subset)
+            held = shard(CM.config5_circuits(args.eval_circuits, seed=777))
+            log(f"[bench] planner: timing {len(held)} held-out circuits in every form")
+            recs = CM.time_forms(mlp, held, states, reps=3, log=log)
+            with open(os.path.join(ddir, f"eval_{args.shard}.json"), "w") as f:
+                json.dump(recs, f)
+        if "report" in stages:
+            recs = []
+            for name in sorted(os.listdir(ddir)):
+                if name.startswith("eval_"):
+                    with open(os.path.join(ddir, name)) as f:
+                        recs += json.load(f)
+            res = CM.summarize(recs)
             out["eval"] = res
-            log(f"[bench] greedy {res['greedy_ms']:.0f} ms, learned {res['learned_ms']:.0f} ms, "
-                f"speedup {res['speedup']:.3f}; vs ket's Fuser on {res.get('reference_fuser_circuits')} "
-                f"circuits: {res.get('speedup_vs_reference_fuser')} (one pass per group), "
-                f"{res.get('speedup_vs_reference_fuser_runs')} (exact runs)")
+            log(f"[bench] {res['circuits']} circuits: " +
+                ", ".join(f"{k} {v:.0f} ms" for k, v in res["total_ms"].items()) +
+                f"; learned x{res['speedup_vs_best_fixed']:.3f} vs the best fixed form "
+                f"({res['best_fixed_form']}), x{res['speedup_vs_fallback']:.3f} vs ket's Fuser "
+                f"as exact runs, noise {res['noise']:.3f}")
     finally:
         states.close()
     ev = out.get("eval", {})
     out.update({
-        "value": ev.get("speedup"), "unit": "x (greedy wall time / learned wall time)",
-        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ev.get("learned_ms"),
+        "value": ev.get("speedup_vs_fallback"),
+        "unit": "x (ket's Fuser as exact runs wall time / learned wall time)",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": ev.get("total_ms", {}).get("learned"),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": "planner: random 13-gate-set circuits, 28-32 qubits, depth 40"},
@@ -476,11 +509,14 @@ def main():
     ap.add_argument("--workload", default="qft30",
                     help="qft30 (default), brick33, hea4096, brick20, qftNN")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage", default="all", help="planner workload: train | eval | all")
+    ap.add_argument("--stage", default="all",
+                    help="planner workload: collect | train | eval | report | all")
+    ap.add_argument("--shard", type=int, default=0, help="planner collect/eval: this shard")
+    ap.add_argument("--shards", type=int, default=1, help="planner collect/eval: shard count")
+    ap.add_argument("--data-dir", default="gpurun_out/planner",
+                    help="planner: where collect/eval write and train/report read")
     ap.add_argument("--train-circuits", type=int, default=5000)
     ap.add_argument("--eval-circuits", type=int, default=200)
-    ap.add_argument("--ref-subset", type=int, default=20,
-                    help="planner eval: circuits also run with ket's own Fuser rule")
     ap.add_argument("--model-out", default="cost_model_trained.npz",
                     help="planner train: where the trained cost model is written "
                          "(the shipped one is ket_amd/cost_model.npz)")

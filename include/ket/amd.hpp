@@ -3,7 +3,7 @@
 // Defaults reproduce ket::run value for value: complex128 amplitudes and
 // ket's own fusion rule. The environment can set them too:
 //   KET_AMD_PRECISION = c128 | c64
-//   KET_AMD_FUSION    = reference | none | exact | blocks | learned
+//   KET_AMD_FUSION    = reference | none | exact | runs | blocks | learned
 //   KET_AMD_BLOCK_K   = 2..6 (width of fused blocks; default: the tuned
 //                       width, 4 for complex128, 5 for complex64)
 #pragma once
@@ -25,8 +25,11 @@ enum class Precision { Complex128, Complex64 };
 // None: one pass per gate. Reference: ket's Fuser rule (with complex128,
 // equal to ket::run value for value). Exact: None's arithmetic with
 // consecutive passes applied on LDS tiles (equal to None bit for bit).
-// Blocks, Learned: dense fused blocks (rounding differs).
-enum class Fusion { None, Reference, Blocks, Learned, Exact };
+// Runs: per-gate passes, commuting gates reordered into fewer exact runs.
+// Blocks: dense fused blocks. Learned: per circuit region, the form a cost
+// model predicts fastest (Runs, Reference or Blocks). Rounding differs from
+// ket's in Runs, Blocks and Learned.
+enum class Fusion { None, Reference, Blocks, Learned, Exact, Runs };
 
 struct Options {
   Precision precision = Precision::Complex128;

@@ -71,14 +71,20 @@ typedef enum ket_fusion {
                              tiles unless ket_tuning.exact_runs is 0 (the
                              values do not change) */
   KET_FUSE_BLOCKS = 2,    /* k-qubit packer: dense 2^k blocks (k = block_k) */
-  KET_FUSE_LEARNED = 3,   /* packer whose block width and block-or-not
-                             choices a trained cost model picks; without a
-                             model it is KET_FUSE_BLOCKS */
+  KET_FUSE_LEARNED = 3,   /* per region of the circuit, the form (runs,
+                             ket's Fuser, blocks of width 3, 4 or 5) a
+                             trained cost model predicts fastest; without a
+                             model it is KET_FUSE_REFERENCE. block_k is not
+                             used (0 or 2..6 accepted) */
   KET_FUSE_BLOCKS_ALL = 4, /* the packer with every group emitted as a block
                              (cost-model training data, batched sweeps) */
-  KET_FUSE_EXACT = 5      /* KET_FUSE_NONE's per-gate arithmetic with
+  KET_FUSE_EXACT = 5,     /* KET_FUSE_NONE's per-gate arithmetic with
                              consecutive passes grouped into exact runs on
                              LDS tiles: equal to KET_FUSE_NONE bit for bit */
+  KET_FUSE_RUNS = 6       /* per-gate passes reordered (commuting gates
+                             pulled forward) into as few exact runs as a
+                             greedy packer finds: per-gate arithmetic in a
+                             different order, equal within rounding */
 } ket_fusion;
 
 typedef struct ket_sv ket_sv; /* a state vector resident on one GPU */
@@ -137,12 +143,12 @@ int ket_sv_apply_matrix(ket_sv* sv, int k, const int* qubits,
                         const double* mat);
 
 /* ---- circuits: apply_circuit + Fuser (src/simulator.cpp:400-517) ---- */
-/* block_k: largest fused block for KET_FUSE_BLOCKS/LEARNED (2..6; 0: 4 for
+/* block_k: largest fused block for KET_FUSE_BLOCKS (2..6; 0: 4 for
  * complex128, 5 for complex64). stats may be NULL. The state keeps its last
  * plan: the same records with the same fusion and block_k (and no launch
- * tuning change since) are not planned again, nor their tables uploaded
- * again while the state's device pool still holds them; KET_FUSE_LEARNED
- * plans afresh every call. */
+ * tuning change since; for KET_FUSE_LEARNED, no change of the planner model
+ * or its parameters either) are not planned again, nor their tables
+ * uploaded again while the state's device pool still holds them. */
 int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates,
                          int fusion, int block_k, ket_run_stats* stats);
 
@@ -283,8 +289,8 @@ int ket_mlp_gradients(ket_mlp* m, const float* x, const float* t, size_t n,
 /* One Adam step (beta1 0.9, beta2 0.999) on the batch; *loss before it. */
 int ket_mlp_train_step(ket_mlp* m, const float* x, const float* t, size_t n,
                        float lr, float* loss);
-/* KET_FUSE_LEARNED plans with this model (NULL: greedy packing). The model
- * must outlive its use; destroying it unsets it. */
+/* KET_FUSE_LEARNED plans with this model (NULL: ket's Fuser as exact runs).
+ * The model must outlive its use; destroying it unsets it. */
 int ket_mlp_use_for_planning(ket_mlp* m);
 
 /* ---- launch shape and fusion width (the autotuner's search space, per
@@ -318,13 +324,13 @@ typedef struct ket_tuning {
                                the two lanes of a pair swapping theirs (0:
                                never; 64: always; default 16 complex128, 4
                                complex64) */
-  int32_t block_k;          /* fused block width of KET_FUSE_BLOCKS and
-                               KET_FUSE_LEARNED calls (and batches) that pass
+  int32_t block_k;          /* fused block width of KET_FUSE_BLOCKS calls
+                               (and batches) that pass
                                block_k = 0: 2..6, or 0 for the built-in 4
                                (complex128) / 5 (complex64) */
-  int32_t exact_runs;       /* KET_FUSE_REFERENCE, and the per-gate passes
-                               KET_FUSE_BLOCKS / KET_FUSE_LEARNED leave
-                               between blocks, group consecutive passes into
+  int32_t exact_runs;       /* KET_FUSE_REFERENCE, KET_FUSE_LEARNED and the
+                               per-gate passes KET_FUSE_BLOCKS leaves
+                               between blocks group consecutive passes into
                                exact runs on LDS tiles: 1 (default) or 0 (one
                                launch per pass) */
   int32_t run_low_bits;     /* exact runs: the lowest index bits every tile

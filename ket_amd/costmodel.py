@@ -2,18 +2,25 @@
 
 Data: pass timings the simulator records itself (ket_sv_profile_circuit: one
 HIP event pair around every launch, plus the pass's feature vector). Each
-training circuit contributes a window of consecutive gates, profiled as
-per-gate passes and as forced blocks of width 3, 4 and 5, so the model sees
-both options of every decision the planner makes. Timing whole 28-32-qubit
-circuits would take hours; timing windows of them samples the same passes.
+training circuit contributes two windows of consecutive gates from one
+random start: 8 gates profiled as per-gate passes, as per-gate exact runs
+and as forced blocks of width 3, 4 and 5, and 48 gates (the learned
+planner's smallest region) profiled in the run forms it weighs, the run
+packer (KET_FUSE_RUNS) and ket's Fuser as exact runs. So the model sees
+every kind of pass the planner's candidate plans hold. Timing whole
+28-32-qubit circuits would take hours; timing windows of them samples the
+same passes.
 
 Model: the 16-64-64-1 MLP of include/ket_amd.h, trained on the GPU with Adam
 on log(milliseconds).
 
-Evaluation: held-out circuits of the same distribution run whole with the
-greedy packer (KET_FUSE_BLOCKS, k = 5) and with the learned planner
-(KET_FUSE_LEARNED); wall time from HIP events, plus the model's error on the
-held-out circuits' passes.
+Evaluation: held-out circuits of the same distribution run whole in every
+form: the greedy block packer (KET_FUSE_BLOCKS, k = 5), ket's Fuser as exact
+runs (KET_FUSE_REFERENCE, the fallback the learned planner is judged
+against), the run packer (KET_FUSE_RUNS) and the learned planner
+(KET_FUSE_LEARNED). Each form runs once cold (planning included; its
+planning time is recorded) and then `reps` times with its plan cached; a
+circuit's time in a form is the median of those. Wall time from HIP events.
 """
 from __future__ import annotations
 
@@ -29,6 +36,15 @@ from .gates import Circuit
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "cost_model.npz")
+
+# name -> (fusion mode, block_k); "learned" plans with the model
+FORMS = {
+    "greedy_blocks": (N.FUSE_BLOCKS, 5),
+    "fuser_runs": (N.FUSE_REFERENCE, 5),
+    "packed_runs": (N.FUSE_RUNS, 5),
+    "learned": (N.FUSE_LEARNED, 5),
+}
+FALLBACK = "fuser_runs"
 
 
 def window(c: Circuit, start: int, length: int) -> Circuit:
@@ -55,25 +71,33 @@ class States:
         self.sv.clear()
 
 
-def collect(circuits, states: States, window_len: int = 12, seed: int = 0, log=None):
+SHORT_FORMS = ((N.FUSE_NONE, 5), (N.FUSE_EXACT, 5), (N.FUSE_BLOCKS_ALL, 3),
+               (N.FUSE_BLOCKS_ALL, 4), (N.FUSE_BLOCKS_ALL, 5))
+LONG_FORMS = ((N.FUSE_RUNS, 5), (N.FUSE_REFERENCE, 5))
+
+
+def collect(circuits, states: States, short_len: int = 8, long_len: int = 48, seed: int = 0,
+            log=None):
+    """Feature rows and log(ms) of every pass of both windows of each
+    circuit, in every form of SHORT_FORMS / LONG_FORMS."""
     rng = np.random.default_rng(seed)
     X, y = [], []
     t0 = time.perf_counter()
     for i, c in enumerate(circuits):
         total = c.n_unitary()
-        start = int(rng.integers(0, max(1, total - window_len)))
-        sub = window(c, start, window_len)
+        start = int(rng.integers(0, max(1, total - long_len)))
         sv = states.get(c.n_qubits)
-        for fusion, k in ((N.FUSE_NONE, 5), (N.FUSE_BLOCKS_ALL, 3), (N.FUSE_BLOCKS_ALL, 4),
-                          (N.FUSE_BLOCKS_ALL, 5)):
-            for r in sv.profile_circuit(sub, fusion, k):
-                if r["ms"] > 0:
-                    X.append(r["features"])
-                    y.append(math.log(r["ms"]))
+        for length, forms in ((short_len, SHORT_FORMS), (long_len, LONG_FORMS)):
+            sub = window(c, start, length)
+            for fusion, k in forms:
+                for r in sv.profile_circuit(sub, fusion, k):
+                    if r["ms"] > 0:
+                        X.append(r["features"])
+                        y.append(math.log(r["ms"]))
         if log and (i + 1) % 250 == 0:
             log(f"[costmodel] timed {i + 1}/{len(circuits)} windows, {len(y)} passes, "
                 f"{time.perf_counter() - t0:.0f} s")
-    return np.array(X, np.float32), np.array(y, np.float32)
+    return np.array(X, np.float32).reshape(-1, N.MLP_INPUTS), np.array(y, np.float32)
 
 
 def train(X, y, epochs: int = 60, batch: int = 1024, lr: float = 3e-3, seed: int = 0, log=None):
@@ -118,52 +142,72 @@ def time_circuit(sv: N.StateVector, c: Circuit, fusion: int, k: int) -> float:
     return sv.event_elapsed(0, 1)
 
 
-def evaluate(mlp: N.MLP, circuits, states: States, log=None, ref_subset: int = 0) -> dict:
-    """Learned planner vs the greedy packer (k = 5, bytes rule) on whole
-    circuits, the model's relative error on the learned plans' passes, and,
-    on the first `ref_subset` circuits, learned vs ket's own greedy Fuser
-    (KET_FUSE_REFERENCE, the reference's fusion rule), run as the reference
-    runs it (one pass per fused group) and as exact runs on LDS tiles."""
-    tg, tl, errs, tr, trr, tlr = [], [], [], [], [], []
-    for i, c in enumerate(circuits):
-        sv = states.get(c.n_qubits)
+def time_forms(mlp: N.MLP, circuits, states: States, reps: int = 3, log=None) -> list:
+    """Per circuit and form: the cold time (planning included), its planning
+    time, the median and spread of `reps` cached repeats, launches and
+    blocks; for the learned form also the model's relative error on the
+    total of its plan's passes."""
+    recs = []
+    mlp.use_for_planning(True)
+    try:
+        for i, c in enumerate(circuits):
+            sv = states.get(c.n_qubits)
+            row = {"n": c.n_qubits, "gates": c.n_unitary()}
+            for name, (fusion, k) in FORMS.items():
+                cold = time_circuit(sv, c, fusion, k)
+                st = dict(sv.last_stats)
+                ts = [time_circuit(sv, c, fusion, k) for _ in range(reps)]
+                med = float(np.median(ts))
+                row[name] = {"cold_ms": cold, "plan_ms": st["plan_seconds"] * 1e3, "ms": med,
+                             "spread": float((max(ts) - min(ts)) / med),
+                             "launches": int(st["launches"]), "blocks": int(st["blocks"])}
+            sv.set_basis(0)
+            prof = sv.profile_circuit(c, N.FUSE_LEARNED, 5)
+            feats = np.array([r["features"] for r in prof], np.float32)
+            meas = np.array([r["ms"] for r in prof])
+            pred = np.exp(mlp.forward(feats))
+            row["learned"]["model_rel_error"] = float(abs(pred.sum() - meas.sum()) / meas.sum())
+            recs.append(row)
+            if log and (i + 1) % 10 == 0:
+                tot = {f: sum(r[f]["ms"] for r in recs) for f in FORMS}
+                log(f"[costmodel] timed {i + 1}/{len(circuits)} circuits: " +
+                    ", ".join(f"{f} {v:.0f} ms" for f, v in tot.items()))
+    finally:
         mlp.use_for_planning(False)
-        time_circuit(sv, c, N.FUSE_BLOCKS, 5)  # warm
-        tg.append(time_circuit(sv, c, N.FUSE_BLOCKS, 5))
-        if i < ref_subset:
-            with N.tuning(sv.precision, exact_runs=0):
-                time_circuit(sv, c, N.FUSE_REFERENCE, 5)  # warm
-                tr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
-            with N.tuning(sv.precision, exact_runs=1):
-                time_circuit(sv, c, N.FUSE_REFERENCE, 5)
-                trr.append(time_circuit(sv, c, N.FUSE_REFERENCE, 5))
-        mlp.use_for_planning(True)
-        tl.append(time_circuit(sv, c, N.FUSE_LEARNED, 5))
-        if i < ref_subset:
-            tlr.append(tl[-1])
-        sv.set_basis(0)
-        recs = sv.profile_circuit(c, N.FUSE_LEARNED, 5)
-        feats = np.array([r["features"] for r in recs], np.float32)
-        pred = np.exp(mlp.forward(feats))
-        meas = np.array([r["ms"] for r in recs])
-        errs.append(float(abs(pred.sum() - meas.sum()) / meas.sum()))
-        if log and (i + 1) % 25 == 0:
-            log(f"[costmodel] evaluated {i + 1}/{len(circuits)}: greedy {sum(tg):.0f} ms, "
-                f"learned {sum(tl):.0f} ms")
-    mlp.use_for_planning(False)
-    tg, tl = np.array(tg), np.array(tl)
-    out = {"circuits": len(circuits), "greedy_ms": float(tg.sum()), "learned_ms": float(tl.sum()),
-           "speedup": float(tg.sum() / tl.sum()),
-           "per_circuit_speedup_median": float(np.median(tg / tl)),
-           "learned_faster_fraction": float(np.mean(tl < tg)),
-           "model_total_time_rel_error_median": float(np.median(errs))}
-    if tr:
-        out.update({"reference_fuser_circuits": len(tr), "reference_fuser_ms": float(sum(tr)),
-                    "reference_fuser_runs_ms": float(sum(trr)),
-                    "learned_ms_same_circuits": float(sum(tlr)),
-                    "speedup_vs_reference_fuser": float(sum(tr) / sum(tlr)),
-                    "speedup_vs_reference_fuser_runs": float(sum(trr) / sum(tlr))})
-    return out
+    return recs
+
+
+def summarize(recs: list) -> dict:
+    """Totals per form, the learned planner against the best non-learned
+    form (the one with the lowest total), the fallback and the greedy
+    packer, and the timing noise (90th percentile of the repeats' relative
+    spread)."""
+    fixed = [f for f in FORMS if f != "learned"]
+    tot = {f: float(sum(r[f]["ms"] for r in recs)) for f in FORMS}
+    cold = {f: float(sum(r[f]["cold_ms"] for r in recs)) for f in FORMS}
+    plan = {f: float(np.mean([r[f]["plan_ms"] for r in recs])) for f in FORMS}
+    best = min(fixed, key=tot.get)
+    per = np.array([min(r[f]["ms"] for f in fixed) / r["learned"]["ms"] for r in recs])
+    spreads = [r[f]["spread"] for r in recs for f in FORMS]
+    return {
+        "circuits": len(recs), "total_ms": tot, "cold_total_ms": cold, "plan_ms_mean": plan,
+        "best_fixed_form": best,
+        "speedup_vs_best_fixed": tot[best] / tot["learned"],
+        "speedup_vs_fallback": tot[FALLBACK] / tot["learned"],
+        "speedup_vs_greedy_blocks": tot["greedy_blocks"] / tot["learned"],
+        "cold_speedup_vs_fallback": cold[FALLBACK] / cold["learned"],
+        "per_circuit_vs_fastest_fixed_median": float(np.median(per)),
+        "learned_fastest_fraction": float(np.mean(per >= 1.0)),
+        "noise": float(np.percentile(spreads, 90)),
+        "learned_launches_mean": float(np.mean([r["learned"]["launches"] for r in recs])),
+        "learned_blocks_mean": float(np.mean([r["learned"]["blocks"] for r in recs])),
+        "model_total_time_rel_error_median":
+            float(np.median([r["learned"]["model_rel_error"] for r in recs])),
+    }
+
+
+def evaluate(mlp: N.MLP, circuits, states: States, reps: int = 3, log=None) -> dict:
+    return summarize(time_forms(mlp, circuits, states, reps, log))
 
 
 def config5_circuits(count: int, seed: int, lo: int = 28, hi: int = 32, depth: int = 40):

@@ -103,6 +103,7 @@ int fusion_code(amd::Fusion f) {
     case amd::Fusion::Blocks: return KET_FUSE_BLOCKS;
     case amd::Fusion::Learned: return KET_FUSE_LEARNED;
     case amd::Fusion::Exact: return KET_FUSE_EXACT;
+    case amd::Fusion::Runs: return KET_FUSE_RUNS;
     default: return KET_FUSE_REFERENCE;
   }
 }
@@ -127,6 +128,7 @@ void init_from_env() {
     else if (!std::strcmp(f, "blocks")) g_opt.fusion = amd::Fusion::Blocks;
     else if (!std::strcmp(f, "learned")) g_opt.fusion = amd::Fusion::Learned;
     else if (!std::strcmp(f, "exact")) g_opt.fusion = amd::Fusion::Exact;
+    else if (!std::strcmp(f, "runs")) g_opt.fusion = amd::Fusion::Runs;
     else g_opt.fusion = amd::Fusion::Reference;
   }
   if (const char* k = std::getenv("KET_AMD_BLOCK_K")) {

@@ -1,9 +1,12 @@
 # Host planner builds under sanitizers, on the CPU (g++ only; no GPU code):
 #   _san/test_runs_asan        tests/cpp/test_runs.cpp, ASan + UBSan
+#   _san/test_runs_tsan        the same with TSan (the learned planner's
+#                              parallel candidate planning)
 #   _san/test_batch_plan_asan  tests/cpp/test_batch_plan.cpp, ASan + UBSan
 #   _san/test_batch_plan_tsan  the same with TSan (16-thread planning)
 # The planner sources are compiled in directly; tests/cpp/san_glue.cpp stands
-# in for the GPU model inference (reporting no model).
+# in for the GPU model inference (no model, or a fixed stand-in function
+# when KET_SAN_STUB_MODEL is set).
 #   make -C ket_amd/cpp -f san.mk
 HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
 ROOT := $(abspath $(HERE)../..)
@@ -18,11 +21,16 @@ PLANNER := $(CSRC)/plan.cpp $(CSRC)/tuning.cpp $(ROOT)/tests/cpp/san_glue.cpp
 DEPS := $(PLANNER) $(wildcard $(CSRC)/*.hpp) $(ROOT)/include/ket_amd.h
 
 .PHONY: all clean
-all: $(OUT)/test_runs_asan $(OUT)/test_batch_plan_asan $(OUT)/test_batch_plan_tsan
+all: $(OUT)/test_runs_asan $(OUT)/test_runs_tsan $(OUT)/test_batch_plan_asan \
+  $(OUT)/test_batch_plan_tsan
 
 $(OUT)/test_runs_asan: $(ROOT)/tests/cpp/test_runs.cpp $(DEPS)
 	@mkdir -p $(OUT)
-	$(CXX) $(BASE) $(ASAN) -o $@ $< $(PLANNER)
+	$(CXX) $(BASE) $(ASAN) -pthread -o $@ $< $(PLANNER)
+
+$(OUT)/test_runs_tsan: $(ROOT)/tests/cpp/test_runs.cpp $(DEPS)
+	@mkdir -p $(OUT)
+	$(CXX) $(BASE) $(TSAN) -pthread -o $@ $< $(PLANNER)
 
 $(OUT)/test_batch_plan_asan: $(ROOT)/tests/cpp/test_batch_plan.cpp $(CSRC)/batch_plan.cpp $(DEPS)
 	@mkdir -p $(OUT)

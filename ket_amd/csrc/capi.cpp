@@ -59,13 +59,15 @@ struct ket_sv {
   ket_run_stats last{};
   uint64_t pool_uploads = 0;  // uploads into `pool` so far
   // The last circuit's plan, reused when the same records arrive again with
-  // the same fusion mode, block width and launch tuning (a repeated step
-  // neither replans nor, while the device pool still holds it, re-uploads).
+  // the same fusion mode, block width, launch tuning and (learned plans)
+  // planner model (a repeated step neither replans nor, while the device
+  // pool still holds it, re-uploads).
   struct {
     bool valid = false;
     std::vector<ket_gate> gates;
     int fusion = 0, block_k = 0;
     uint64_t tuning = 0;
+    uint64_t model = 0;
     uint64_t upload = 0;  // pool_uploads right after its pool was uploaded
     ketplan::Plan plan;
   } cached;
@@ -508,12 +510,10 @@ int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int 
   if (n_gates && !gates) return fail(KET_ERR_ARG, "ket_amd: null gate list");
   const auto t0 = std::chrono::steady_clock::now();
   auto& c = sv->cached;
-  // The learned planner's model can change between calls: its plans are
-  // made afresh.
-  const bool cacheable = fusion != KET_FUSE_LEARNED;
   const uint64_t tuning = ketk::tuning_generation();
-  if (cacheable && c.valid && c.fusion == fusion && c.block_k == block_k && c.tuning == tuning &&
-      c.gates.size() == n_gates &&
+  const uint64_t model = fusion == KET_FUSE_LEARNED ? ketplan::planner_model_generation() : 0;
+  if (c.valid && c.fusion == fusion && c.block_k == block_k && c.tuning == tuning &&
+      c.model == model && c.gates.size() == n_gates &&
       (n_gates == 0 || std::memcmp(c.gates.data(), gates, n_gates * sizeof(ket_gate)) == 0)) {
     const double plan_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -532,11 +532,12 @@ int ket_sv_apply_circuit(ket_sv* sv, const ket_gate* gates, size_t n_gates, int 
   if (stats) *stats = sv->last;
   c.valid = false;
   const int st = run_plan(sv, plan);
-  if (st == KET_OK && cacheable) {
+  if (st == KET_OK) {
     c.gates.assign(gates, gates + n_gates);
     c.fusion = fusion;
     c.block_k = block_k;
     c.tuning = tuning;
+    c.model = model;
     c.upload = sv->pool_uploads;
     c.plan = std::move(plan);
     c.valid = true;

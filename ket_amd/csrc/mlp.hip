@@ -11,6 +11,7 @@
 //   k_loss         mean squared error of the batch
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -229,6 +230,7 @@ int gradients(ket_mlp* m, const float* x, const float* t, size_t n) {
 }
 
 ket_mlp* g_planner_model = nullptr;
+std::atomic<uint64_t> g_model_generation{1};
 
 }  // namespace
 
@@ -247,6 +249,8 @@ bool predict_log_ms(const std::vector<float>& feats, std::vector<float>* out) {
     return false;
   return hipStreamSynchronize(m->stream) == hipSuccess;
 }
+bool planner_model_set() { return g_planner_model != nullptr; }
+uint64_t planner_model_generation() { return g_model_generation.load(); }
 }  // namespace ketplan
 
 extern "C" {
@@ -284,6 +288,7 @@ int ket_mlp_create(uint64_t seed, ket_mlp** out) {
 int ket_mlp_destroy(ket_mlp* m) {
   if (!m) return KET_OK;
   if (g_planner_model == m) g_planner_model = nullptr;
+  ++g_model_generation;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (float* p : {m->params, m->grad, m->m, m->v, m->scratch})
     if (p) (void)hipFree(p);
@@ -297,6 +302,7 @@ int ket_mlp_set_params(ket_mlp* m, const float* params, size_t n) {
     return fail(KET_ERR_ARG, "ket_amd: set_params needs KET_MLP_PARAMS floats");
   KET_HIP_TRY(hipMemcpy(m->params, params, n * sizeof(float), hipMemcpyHostToDevice),
               "hipMemcpy(params)");
+  ++g_model_generation;
   return KET_OK;
 }
 
@@ -338,6 +344,7 @@ int ket_mlp_train_step(ket_mlp* m, const float* x, const float* t, size_t n, flo
   if (!m || !x || !t || n == 0) return fail(KET_ERR_ARG, "ket_amd: bad mlp batch");
   if (int st = gradients(m, x, t, n)) return st;
   ++m->step;
+  ++g_model_generation;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
   k_adam<<<(NPARAM + 255) / 256, 256, 0, m->stream>>>(m->params, m->grad, m->m, m->v, lr, b1, b2,
@@ -353,6 +360,7 @@ int ket_mlp_train_step(ket_mlp* m, const float* x, const float* t, size_t n, flo
 
 int ket_mlp_use_for_planning(ket_mlp* m) {
   g_planner_model = m;
+  ++g_model_generation;
   return KET_OK;
 }
 

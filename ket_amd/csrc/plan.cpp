@@ -19,12 +19,14 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 
 #include "hostmath.hpp"
+#include "parallel.hpp"
 
 namespace ketplan {
 namespace {
@@ -446,12 +448,20 @@ std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, in
 
 }  // namespace
 
+namespace {
+
+// The lowest bits every run tile keeps: HBM rows of 2^low amplitudes.
+int run_low_bits(int precision) {
+  const int tuned = ketk::get_tuning(precision).run_low_bits;
+  return std::min(tuned ? tuned : (precision == 0 ? 3 : 4), ketk::run_tile_bits(precision) - 1);
+}
+
+}  // namespace
+
 void group_runs(Plan& plan, int n, int precision) {
   const int tb = ketk::run_tile_bits(precision);
   if (n < tb) return;
-  // the lowest bits every tile keeps: HBM rows of 2^low amplitudes
-  const int tuned = ketk::get_tuning(precision).run_low_bits;
-  const int low = std::min(tuned ? tuned : (precision == 0 ? 3 : 4), tb - 1);
+  const int low = run_low_bits(precision);
   const uint64_t low_mask = bit(low) - 1;
   std::vector<Op> in;
   in.swap(plan.ops);
@@ -514,6 +524,12 @@ void group_runs(Plan& plan, int n, int precision) {
     o.line_fraction = 1.0;
     o.run_pmask = P;
     o.run_ops = (int)ops.size();
+    o.run_segs = (int)segs.size();
+    for (const auto& r : ops) {
+      o.run_phase_ops += r.kind == ketk::RUN_PHASE;
+      o.run_matrix_ops += r.kind == ketk::RUN_MATRIX;
+      o.run_perm_ops += r.kind == ketk::RUN_SWAP;
+    }
     const double dim = std::ldexp(1.0, n);
     for (std::size_t t = i; t < j; ++t) {
       const Op& x = in[t];
@@ -693,6 +709,14 @@ bool default_decider(const BlockChoice& c) {
   return c.single_sweeps > 1.0;
 }
 
+// Features (all scaled to about [0, 1]): 0-4 and 15 the pass kind (block,
+// dense pair, phase, permutation pair, other exact pass (exact matrix,
+// controlled H), run); 5 width; 6 gates; 7 phase terms; 8, 9 lowest and
+// highest qubit; 10 qubits inside a 128-byte line; 11 line fraction; 12
+// precision; 13 state size; 14 log2 of the pass's algorithmic bytes. A run's
+// width is its register layouts (segments), its gates the passes it
+// applies, its terms its phase passes, slot 8 its exact-matrix passes,
+// slot 10 its permutation passes and its highest qubit the tile's.
 void op_features(const Op& o, int n, int precision, float* f) {
   for (int i = 0; i < KET_MLP_INPUTS; ++i) f[i] = 0.f;
   const int L = line_bits(precision);
@@ -717,22 +741,28 @@ void op_features(const Op& o, int n, int precision, float* f) {
       break;
     case Kind::Run:
       qm = o.run_pmask;
-      width = std::min(o.run_ops, 64) / 8;
-      f[4] = 1.f;
+      f[15] = 1.f;
       break;
   }
   const double sweep = 2.0 * (precision == 0 ? 16.0 : 8.0) * (double)(uint64_t{1} << n);
-  f[5] = width / 6.f;
-  f[6] = std::min(o.n_gates, 128) / 32.f;
-  f[7] = o.n_terms / 64.f;
-  f[8] = qm ? __builtin_ctzll(qm) / 32.f : 0.f;
+  if (o.kind == Kind::Run) {
+    f[5] = o.run_segs / 8.f;
+    f[6] = std::min(o.run_ops, 128) / 32.f;
+    f[7] = o.run_phase_ops / 32.f;
+    f[8] = o.run_matrix_ops / 8.f;
+    f[10] = o.run_perm_ops / 32.f;
+  } else {
+    f[5] = width / 6.f;
+    f[6] = std::min(o.n_gates, 128) / 32.f;
+    f[7] = o.n_terms / 64.f;
+    f[8] = qm ? __builtin_ctzll(qm) / 32.f : 0.f;
+    f[10] = __builtin_popcountll(qm & (bit(L) - 1)) / 4.f;
+  }
   f[9] = qm ? (63 - __builtin_clzll(qm)) / 32.f : 0.f;
-  f[10] = __builtin_popcountll(qm & (bit(L) - 1)) / 4.f;
   f[11] = (float)o.line_fraction;
   f[12] = (float)precision;
   f[13] = (n - 24) / 8.f;
   f[14] = (float)(std::log2(sweep * o.line_fraction) / 40.0);
-  f[15] = 1.f;
 }
 
 void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
@@ -829,8 +859,14 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
         block.flops_per_amp = 8.0 * (double)(1 << k);
       }
       block.n_gates = (int)order.size();
-      if (decide) (void)decide(ch, block, singles);
+      const bool use_block =
+          force_blocks || (decide ? decide(ch, block, singles) : default_decider(ch));
+      if (!use_block) {
+        for (const Op& o : singles) plan->ops.push_back(o);
+        continue;
+      }
       plan->ops.push_back(block);
+      ++plan->blocks;
       continue;
     }
     std::vector<kethost::GateRef> refs;
@@ -861,77 +897,213 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
   }
 }
 
-// KET_FUSE_LEARNED: for each candidate block width, pack the circuit once
-// recording every group's block pass and its gates' own passes, predict all
-// of their times with the cost model in one GPU inference, keep the width
-// whose predicted total (the cheaper option per group) is lowest, and return
-// a decider that replays the per-group choices. Without a model the greedy
-// bytes rule and the requested width stay.
-void learned_choice(const ket_gate* gates, std::size_t n_gates, int n, int precision,
-                    int* block_k, BlockDecider* decider) {
-  struct Group {
-    std::size_t block_row, single_row, n_single;
+void pack_runs(const ket_gate* gates, const std::vector<std::size_t>& run, int n,
+               int precision, Plan* plan) {
+  const uint64_t low_mask = bit(run_low_bits(precision)) - 1;
+  const int cap = ketk::run_tile_bits(precision) - run_low_bits(precision);
+  const uint64_t all = n >= 64 ? ~uint64_t{0} : (bit(n) - 1);
+  // q: the gate's qubits; tile: the bits its pass moves (a run's tile must
+  // hold them); moves: the bits on which it is not diagonal. Gate a commutes
+  // past gate b when neither moves a bit of the other (controls and
+  // diagonal gates move none).
+  struct Pending {
+    uint64_t q, tile, moves;
+    Op op;
   };
-  std::vector<float> feats;
-  std::vector<std::vector<Group>> groups;
-  std::vector<int> widths;
-  for (int k : {3, 4, 5}) {
-    if (k > n) continue;
-    widths.push_back(k);
-    groups.emplace_back();
-    std::vector<Group>& gs = groups.back();
-    Plan scratch;
-    BlockDecider rec = [&](const BlockChoice&, const Op& block, const std::vector<Op>& singles) {
-      Group g{feats.size() / KET_MLP_INPUTS, 0, singles.size()};
-      feats.resize(feats.size() + KET_MLP_INPUTS);
-      op_features(block, n, precision, &feats[feats.size() - KET_MLP_INPUTS]);
-      g.single_row = feats.size() / KET_MLP_INPUTS;
-      for (const Op& o : singles) {
-        feats.resize(feats.size() + KET_MLP_INPUTS);
-        op_features(o, n, precision, &feats[feats.size() - KET_MLP_INPUTS]);
-      }
-      gs.push_back(g);
-      return true;
-    };
-    std::vector<std::size_t> run;
-    for (std::size_t i = 0; i <= n_gates; ++i) {
-      if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
-        if (!run.empty()) pack_blocks(gates, run, n, precision, k, &scratch, rec, false, true);
-        run.clear();
+  std::vector<Pending> rem;
+  rem.reserve(run.size());
+  std::vector<Term> terms;
+  for (std::size_t gi : run) {
+    const ket_gate& g = gates[gi];
+    Pending x{0, 0, 0, op_for_gate(g, n, precision)};
+    for (int j = 0; j < g.n_qubits; ++j) x.q |= bit(g.qubits[j]);
+    const int64_t mb = moved_bits(x.op);
+    x.tile = mb > 0 ? (uint64_t)mb : 0;
+    terms.clear();
+    x.moves = diag_terms(g, &terms) ? 0 : x.tile;
+    rem.push_back(x);
+  }
+  // Bounds the scan per group (the planning cost is groups x window).
+  constexpr std::size_t kWindow = 4096;
+  std::vector<char> taken;
+  while (!rem.empty()) {
+    taken.assign(rem.size(), 0);
+    uint64_t moved = 0, skipped_q = 0, skipped_moves = 0;
+    const std::size_t end = std::min(rem.size(), kWindow);
+    for (std::size_t i = 0; i < end; ++i) {
+      const Pending& x = rem[i];
+      const bool passes = !(x.moves & skipped_q) && !(x.q & skipped_moves);
+      const uint64_t m2 = moved | x.tile;
+      if (passes && __builtin_popcountll(m2 & ~low_mask) <= cap) {
+        moved = m2;
+        taken[i] = 1;
+        plan->ops.push_back(x.op);
         continue;
       }
-      run.push_back(i);
+      skipped_q |= x.q;
+      skipped_moves |= x.moves;
+      if (skipped_moves == all) break;  // nothing later can pass
     }
+    std::size_t w = 0;
+    for (std::size_t i = 0; i < rem.size(); ++i)
+      if (!taken[i]) rem[w++] = rem[i];
+    rem.resize(w);
   }
-  std::vector<float> pred;
-  if (widths.empty() || !predict_log_ms(feats, &pred)) return;
-  double best = 1e300;
-  std::size_t best_w = 0;
-  for (std::size_t w = 0; w < widths.size(); ++w) {
-    double total = 0.0;
-    for (const Group& g : groups[w]) {
-      double singles = 0.0;
-      for (std::size_t j = 0; j < g.n_single; ++j) singles += std::exp((double)pred[g.single_row + j]);
-      total += std::min(std::exp((double)pred[g.block_row]), singles);
-    }
-    if (total < best) {
-      best = total;
-      best_w = w;
-    }
-  }
-  *block_k = widths[best_w];
-  auto choice = std::make_shared<std::vector<char>>();
-  for (const Group& g : groups[best_w]) {
-    double singles = 0.0;
-    for (std::size_t j = 0; j < g.n_single; ++j) singles += std::exp((double)pred[g.single_row + j]);
-    choice->push_back(std::exp((double)pred[g.block_row]) < singles);
-  }
-  auto next = std::make_shared<std::size_t>(0);
-  *decider = [choice, next](const BlockChoice&, const Op&, const std::vector<Op>&) {
-    const std::size_t i = (*next)++;
-    return i < choice->size() ? (*choice)[i] != 0 : false;
-  };
 }
+
+namespace {
+
+// ket's Fuser over the unitary gate indices `run` (structural records have
+// already split the circuit into such stretches).
+void fuser_region(const ket_gate* gates, const std::vector<std::size_t>& run, int n,
+                  int precision, Plan* plan) {
+  std::vector<const ket_gate*> pend;
+  std::vector<int> bq;
+  for (std::size_t gi : run) {
+    const ket_gate& g = gates[gi];
+    bool subset = true;
+    for (int j = 0; j < g.n_qubits; ++j)
+      if (std::find(bq.begin(), bq.end(), g.qubits[j]) == bq.end()) subset = false;
+    if (!pend.empty() && !subset) fuser_flush(pend, bq, n, precision, plan);
+    if (pend.empty()) bq.assign(g.qubits, g.qubits + g.n_qubits);
+    pend.push_back(&g);
+  }
+  fuser_flush(pend, bq, n, precision, plan);
+}
+
+// fn(indices) for each maximal stretch of unitary gates.
+template <class F>
+void for_each_stretch(const ket_gate* gates, std::size_t n_gates, F fn) {
+  std::vector<std::size_t> run;
+  for (std::size_t i = 0; i <= n_gates; ++i) {
+    if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
+      if (!run.empty()) fn(run);
+      run.clear();
+      continue;
+    }
+    run.push_back(i);
+  }
+}
+
+}  // namespace
+
+void plan_region(const ket_gate* gates, const std::vector<std::size_t>& run, int n,
+                 int precision, Form form, Plan* plan, bool features_only) {
+  switch (form) {
+    case Form::Runs:
+      pack_runs(gates, run, n, precision, plan);
+      return;
+    case Form::Fuser:
+      fuser_region(gates, run, n, precision, plan);
+      return;
+    default: {
+      const int k = form == Form::Blocks3 ? 3 : form == Form::Blocks4 ? 4 : 5;
+      pack_blocks(gates, run, n, precision, std::max(1, std::min(k, n)), plan, {}, false,
+                  features_only);
+    }
+  }
+}
+
+namespace {
+
+// KET_FUSE_LEARNED. Each stretch of unitary gates is split into regions: the
+// whole stretch, its halves, quarters and eighths (while a region keeps at
+// least kMinRegion gates), a binary tree in heap order. Every region is
+// planned in every form (the run packer, ket's Fuser, the block packer at
+// widths 3, 4 and 5 with its bytes rule, each followed by the grouping into
+// exact runs as the final plan does it), the cost model predicts the time
+// of all of their passes in one GPU inference, and a region keeps its
+// cheapest form unless its two halves, each planned at its best, cost less.
+// Candidate block passes are planned without matrices (features only). The
+// chosen regions are then planned in full, in order, and grouped into runs
+// once. False when no model is set.
+constexpr std::size_t kMinRegion = 48;
+constexpr int kMaxLevel = 3;
+
+bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precision, Plan* plan) {
+  if (!planner_model_set()) return false;
+  const bool runs = ketk::get_tuning(precision).exact_runs != 0;
+  std::vector<std::vector<std::size_t>> parts;
+  for_each_stretch(gates, n_gates, [&](const std::vector<std::size_t>& r) { parts.push_back(r); });
+  struct Task {
+    std::size_t part, a, b;
+    int node, form;
+    std::vector<float> rows;
+  };
+  std::vector<int> levels(parts.size());
+  std::vector<Task> tasks;
+  for (std::size_t p = 0; p < parts.size(); ++p) {
+    const std::size_t G = parts[p].size();
+    int L = 0;
+    while (L < kMaxLevel && (G >> (L + 1)) >= kMinRegion) ++L;
+    levels[p] = L;
+    for (int l = 0; l <= L; ++l)
+      for (std::size_t j = 0; j < (std::size_t{1} << l); ++j)
+        for (int f = 0; f < kForms; ++f)
+          tasks.push_back({p, (j * G) >> l, ((j + 1) * G) >> l, (int)((1u << l) - 1 + j), f, {}});
+  }
+  ketpar::parallel_for(tasks.size(), [&](std::size_t t) {
+    Task& T = tasks[t];
+    const std::vector<std::size_t> idx(parts[T.part].begin() + T.a, parts[T.part].begin() + T.b);
+    Plan scratch;
+    plan_region(gates, idx, n, precision, (Form)T.form, &scratch, true);
+    if (runs) group_runs(scratch, n, precision);
+    T.rows.resize(scratch.ops.size() * KET_MLP_INPUTS);
+    for (std::size_t i = 0; i < scratch.ops.size(); ++i)
+      op_features(scratch.ops[i], n, precision, &T.rows[i * KET_MLP_INPUTS]);
+  });
+  std::vector<float> feats;
+  for (const Task& T : tasks) feats.insert(feats.end(), T.rows.begin(), T.rows.end());
+  std::vector<float> pred;
+  if (!feats.empty() && !predict_log_ms(feats, &pred)) return false;
+  // cost[part][node][form]
+  std::vector<std::vector<std::array<double, kForms>>> cost(parts.size());
+  for (std::size_t p = 0; p < parts.size(); ++p)
+    cost[p].resize((std::size_t{2} << levels[p]) - 1);
+  std::size_t row = 0;
+  for (const Task& T : tasks) {
+    double c = 0.0;
+    for (std::size_t i = 0; i < T.rows.size() / KET_MLP_INPUTS; ++i) c += std::exp((double)pred[row++]);
+    cost[T.part][T.node][T.form] = c;
+  }
+  for (std::size_t p = 0; p < parts.size(); ++p) {
+    const std::size_t nodes = cost[p].size(), G = parts[p].size();
+    std::vector<double> best(nodes);
+    std::vector<int> choice(nodes);  // a form, or -1: split in halves
+    for (std::size_t h = nodes; h-- > 0;) {
+      int f = 0;
+      for (int x = 1; x < kForms; ++x)
+        if (cost[p][h][x] < cost[p][h][f]) f = x;
+      best[h] = cost[p][h][f];
+      choice[h] = f;
+      if (2 * h + 2 < nodes && best[2 * h + 1] + best[2 * h + 2] < best[h]) {
+        best[h] = best[2 * h + 1] + best[2 * h + 2];
+        choice[h] = -1;
+      }
+    }
+    // In order: node h at level l, position j covers [j G >> l, (j+1) G >> l).
+    std::vector<std::size_t> stack{0};
+    while (!stack.empty()) {
+      const std::size_t h = stack.back();
+      stack.pop_back();
+      if (choice[h] < 0) {
+        stack.push_back(2 * h + 2);
+        stack.push_back(2 * h + 1);
+        continue;
+      }
+      int l = 0;
+      while (((std::size_t{2} << l) - 1) <= h) ++l;
+      const std::size_t j = h - ((std::size_t{1} << l) - 1);
+      const std::vector<std::size_t> idx(parts[p].begin() + ((j * G) >> l),
+                                         parts[p].begin() + (((j + 1) * G) >> l));
+      plan_region(gates, idx, n, precision, (Form)choice[h], plan, false);
+    }
+  }
+  if (runs) group_runs(*plan, n, precision);
+  return true;
+}
+
+}  // namespace
 
 int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
                  int precision, int fusion, int block_k, Plan* out,
@@ -950,47 +1122,33 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
     if (fusion == KET_FUSE_EXACT) group_runs(plan, n, precision);
     return KET_OK;
   }
-  if (fusion == KET_FUSE_REFERENCE) {
-    std::vector<const ket_gate*> pend;
-    std::vector<int> bq;
-    for (std::size_t i = 0; i < n_gates; ++i) {
-      const ket_gate& g = gates[i];
-      if (!kethost::is_unitary(g.type)) {
-        fuser_flush(pend, bq, n, precision, &plan);
-        continue;
-      }
-      bool subset = true;
-      for (int j = 0; j < g.n_qubits; ++j)
-        if (std::find(bq.begin(), bq.end(), g.qubits[j]) == bq.end())
-          subset = false;
-      if (!pend.empty() && !subset) fuser_flush(pend, bq, n, precision, &plan);
-      if (pend.empty()) bq.assign(g.qubits, g.qubits + g.n_qubits);
-      pend.push_back(&g);
+  if (fusion == KET_FUSE_LEARNED) {
+    if (block_k != 0 && (block_k < 2 || block_k > 6)) {
+      *err = "ket_amd: block_k must be 0 or in [2, 6]";
+      return KET_ERR_ARG;
     }
-    fuser_flush(pend, bq, n, precision, &plan);
-    if (ketk::get_tuning(precision).exact_runs) group_runs(plan, n, precision);
+    if (learned_plan(gates, n_gates, n, precision, &plan)) return KET_OK;
+    fusion = KET_FUSE_REFERENCE;  // no model: ket's Fuser, as exact runs
+  }
+  if (fusion == KET_FUSE_REFERENCE || fusion == KET_FUSE_RUNS) {
+    for_each_stretch(gates, n_gates, [&](const std::vector<std::size_t>& run) {
+      if (fusion == KET_FUSE_RUNS) pack_runs(gates, run, n, precision, &plan);
+      else fuser_region(gates, run, n, precision, &plan);
+    });
+    if (fusion == KET_FUSE_RUNS || ketk::get_tuning(precision).exact_runs)
+      group_runs(plan, n, precision);
     return KET_OK;
   }
-  if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_LEARNED || fusion == KET_FUSE_BLOCKS_ALL) {
+  if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_BLOCKS_ALL) {
     if (block_k == 0) block_k = default_block_k(precision);
     if (block_k < 2 || block_k > 6) {
       *err = "ket_amd: block_k must be 0 or in [2, 6]";
       return KET_ERR_ARG;
     }
     if (block_k > n) block_k = n < 1 ? 1 : n;
-    BlockDecider decider;  // empty: bytes rule
-    if (fusion == KET_FUSE_LEARNED) learned_choice(gates, n_gates, n, precision, &block_k, &decider);
-    std::vector<std::size_t> run;
-    for (std::size_t i = 0; i <= n_gates; ++i) {
-      if (i == n_gates || !kethost::is_unitary(gates[i].type)) {
-        if (!run.empty())
-          pack_blocks(gates, run, n, precision, block_k, &plan, decider,
-                      fusion == KET_FUSE_BLOCKS_ALL);
-        run.clear();
-        continue;
-      }
-      run.push_back(i);
-    }
+    for_each_stretch(gates, n_gates, [&](const std::vector<std::size_t>& run) {
+      pack_blocks(gates, run, n, precision, block_k, &plan, {}, fusion == KET_FUSE_BLOCKS_ALL);
+    });
     // The per-gate passes the packer leaves between blocks run as exact
     // runs where they are consecutive (their values do not change).
     if (fusion != KET_FUSE_BLOCKS_ALL && ketk::get_tuning(precision).exact_runs)

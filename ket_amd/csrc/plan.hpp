@@ -28,6 +28,10 @@ struct Op {
   int n_terms = 0;                          // Block: trailing phase terms
   uint64_t run_pmask = 0;                   // Run: the tile positions P
   int run_ops = 0;                          // Run: passes it applies
+  int run_segs = 0;                         // Run: register layouts (segments)
+  int run_phase_ops = 0;                    // Run: phase passes among run_ops
+  int run_matrix_ops = 0;                   // Run: exact-matrix passes among them
+  int run_perm_ops = 0;                     // Run: permutation passes (X, SWAP, ...)
 };
 
 // A diagonal phase term: multiply by exp(i * angle) where all bits of mask
@@ -122,6 +126,10 @@ void op_features(const Op& o, int n, int precision, float* f);
 // Predicted log-milliseconds of feature rows with the planner's model
 // (mlp.hip); false when no model is set.
 bool predict_log_ms(const std::vector<float>& feats, std::vector<float>* out);
+// Whether a planner model is set, and a counter that changes whenever the
+// model or its parameters may have (plans made with it are cached by it).
+bool planner_model_set();
+std::uint64_t planner_model_generation();
 
 // The k-qubit packer (KET_FUSE_BLOCKS) over the unitary gate indices `run`.
 // force_blocks: every packed group becomes a Block pass (one kernel kind
@@ -132,5 +140,25 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
                  int n, int precision, int block_k, Plan* plan,
                  const BlockDecider& decide, bool force_blocks = false,
                  bool features_only = false);
+
+// The run packer (KET_FUSE_RUNS) over the unitary gate indices `run`: each
+// gate's own pass (op_for_gate), reordered so that consecutive passes form
+// as few exact runs as the greedy rule finds. Starting from the first
+// pending gate it takes every later gate that commutes past the gates it
+// leaves behind and whose moved bits still fit one run's tile; a group's
+// gates keep their program order. Reordering commuting passes changes the
+// rounding, not the values (within the fused modes' tolerance); group_runs
+// then turns each group into one run.
+void pack_runs(const ket_gate* gates, const std::vector<std::size_t>& run, int n,
+               int precision, Plan* plan);
+
+// The forms the learned planner chooses between for a region of a circuit.
+enum class Form { Runs, Fuser, Blocks3, Blocks4, Blocks5 };
+constexpr int kForms = 5;
+
+// Appends the passes of `form` over the unitary gate indices `run` to plan
+// (before group_runs). features_only as in pack_blocks.
+void plan_region(const ket_gate* gates, const std::vector<std::size_t>& run, int n,
+                 int precision, Form form, Plan* plan, bool features_only);
 
 }  // namespace ketplan

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libket_amd.so")
 
 C128, C64 = 0, 1
-FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED, FUSE_BLOCKS_ALL, FUSE_EXACT = 0, 1, 2, 3, 4, 5
+FUSE_NONE, FUSE_REFERENCE, FUSE_BLOCKS, FUSE_LEARNED, FUSE_BLOCKS_ALL, FUSE_EXACT, FUSE_RUNS = (
+    0, 1, 2, 3, 4, 5, 6)
 
 STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: MemoryError,
           5: NotImplementedError}

@@ -7,15 +7,23 @@
 // be equal bit for bit. Every record's layouts are checked first
 // (check_record).
 //
-//   test_runs <records.bin> <n> <precision 0|1> <mode: none | reference | features>
-//             [run_low_bits]
+//   test_runs <records.bin> <n> <precision 0|1>
+//             <mode: none | reference | runs | features | learned> [run_low_bits]
 //
 // mode none compares KET_FUSE_NONE with KET_FUSE_EXACT; mode reference
 // compares KET_FUSE_REFERENCE with exact runs off and on. Prints
 // "ok <passes> <runs> <passes in runs> <LDS bank-row conflicts>" or the
-// first difference. Mode
-// features compares the learned planner's feature rows from the packer with
-// and without its matrix-free recording mode.
+// first difference. Mode runs compares KET_FUSE_NONE with KET_FUSE_RUNS
+// (reordered commuting passes: equal within rounding, max |difference| at
+// most 1e-10 (complex128) or 1e-4 (complex64) times the largest amplitude).
+// Mode features compares, for every form the learned planner weighs, the
+// feature rows of the region plans it predicts (block passes without
+// matrices) with those of the plans it then executes. Mode learned plans
+// KET_FUSE_LEARNED twice (with whatever model the build provides) and
+// checks that both plans are equal and every run record well formed;
+// prints "ok learned <passes> <runs> <blocks>".
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -349,12 +357,15 @@ void execute(std::vector<C<T>>& s, const ketplan::Plan& plan, int n, int precisi
 }
 
 template <class T>
-int compare(const std::vector<ket_gate>& g, int n, int precision, bool reference) {
+int compare(const std::vector<ket_gate>& g, int n, int precision, bool reference, bool reorder) {
   ketplan::Plan a, b;
   std::string err;
   ketk::Tuning t = ketk::get_tuning(precision);
   int st;
-  if (reference) {
+  if (reorder) {
+    st = ketplan::plan_circuit(g.data(), g.size(), n, precision, KET_FUSE_NONE, 0, &a, &err);
+    if (!st) st = ketplan::plan_circuit(g.data(), g.size(), n, precision, KET_FUSE_RUNS, 0, &b, &err);
+  } else if (reference) {
     t.exact_runs = 0;
     ketk::set_tuning(precision, t);
     st = ketplan::plan_circuit(g.data(), g.size(), n, precision, KET_FUSE_REFERENCE, 0, &a, &err);
@@ -380,7 +391,19 @@ int compare(const std::vector<ket_gate>& g, int n, int precision, bool reference
   std::vector<C<T>> s2 = s;
   execute(s, a, n, precision);
   execute(s2, b, n, precision);
-  for (std::size_t i = 0; i < s.size(); ++i)
+  if (reorder) {
+    double amax = 0.0, dmax = 0.0;
+    for (std::size_t i = 0; i < s.size(); ++i) {
+      amax = std::max({amax, std::fabs((double)s[i].x), std::fabs((double)s[i].y)});
+      dmax = std::max({dmax, std::fabs((double)s[i].x - (double)s2[i].x),
+                       std::fabs((double)s[i].y - (double)s2[i].y)});
+    }
+    if (!(dmax <= (precision == 0 ? 1e-10 : 1e-4) * amax)) {
+      std::printf("runs differ: max |diff| %g, max |amp| %g\n", dmax, amax);
+      return 1;
+    }
+  }
+  for (std::size_t i = 0; i < s.size() && !reorder; ++i)
     if (std::memcmp(&s[i], &s2[i], sizeof(C<T>)) != 0) {
       std::printf("mismatch at %zu: (%a, %a) vs (%a, %a)\n", i, (double)s[i].x, (double)s[i].y,
                   (double)s2[i].x, (double)s2[i].y);
@@ -397,30 +420,25 @@ int compare(const std::vector<ket_gate>& g, int n, int precision, bool reference
   return 0;
 }
 
-// The learned planner's recording packs (features only, no matrices) must
-// describe every pass exactly as the full packer's passes.
+// The learned planner predicts the cost of region plans whose block passes
+// carry no matrices; every form's rows must equal those of the full plan.
 int features(const std::vector<ket_gate>& g, int n, int precision) {
   std::vector<std::size_t> run;
   for (std::size_t i = 0; i < g.size(); ++i) run.push_back(i);
-  for (int k : {3, 4, 5}) {
+  for (int f = 0; f < ketplan::kForms; ++f) {
     std::vector<std::vector<float>> rows[2];
     for (int fo = 0; fo < 2; ++fo) {
-      ketplan::Plan scratch;
-      ketplan::BlockDecider rec = [&](const ketplan::BlockChoice&, const ketplan::Op& block,
-                                      const std::vector<ketplan::Op>& singles) {
-        std::vector<float> f(KET_MLP_INPUTS);
-        ketplan::op_features(block, n, precision, f.data());
-        rows[fo].push_back(f);
-        for (const auto& o : singles) {
-          ketplan::op_features(o, n, precision, f.data());
-          rows[fo].push_back(f);
-        }
-        return true;
-      };
-      ketplan::pack_blocks(g.data(), run, n, precision, k, &scratch, rec, false, fo == 1);
+      ketplan::Plan p;
+      ketplan::plan_region(g.data(), run, n, precision, (ketplan::Form)f, &p, fo == 1);
+      ketplan::group_runs(p, n, precision);
+      for (const auto& o : p.ops) {
+        std::vector<float> x(KET_MLP_INPUTS);
+        ketplan::op_features(o, n, precision, x.data());
+        rows[fo].push_back(x);
+      }
     }
     if (rows[0] != rows[1]) {
-      std::printf("feature rows differ at k=%d (%zu vs %zu rows)\n", k, rows[0].size(),
+      std::printf("feature rows differ in form %d (%zu vs %zu rows)\n", f, rows[0].size(),
                   rows[1].size());
       return 1;
     }
@@ -429,11 +447,52 @@ int features(const std::vector<ket_gate>& g, int n, int precision) {
   return 0;
 }
 
+int learned(const std::vector<ket_gate>& g, int n, int precision) {
+  ketplan::Plan p[2];
+  std::string err;
+  for (auto& q : p)
+    if (ketplan::plan_circuit(g.data(), g.size(), n, precision, KET_FUSE_LEARNED, 0, &q, &err)) {
+      std::fprintf(stderr, "plan: %s\n", err.c_str());
+      return 2;
+    }
+  if (p[0].pool != p[1].pool || p[0].ops.size() != p[1].ops.size()) {
+    std::printf("learned plans differ\n");
+    return 1;
+  }
+  int runs = 0, blocks = 0;
+  for (std::size_t i = 0; i < p[0].ops.size(); ++i) {
+    std::vector<float> a(KET_MLP_INPUTS), b(KET_MLP_INPUTS);
+    ketplan::op_features(p[0].ops[i], n, precision, a.data());
+    ketplan::op_features(p[1].ops[i], n, precision, b.data());
+    if (a != b) {
+      std::printf("learned plans differ at pass %zu\n", i);
+      return 1;
+    }
+    const auto& o = p[0].ops[i];
+    if (o.kind == ketplan::Kind::Run) {
+      ++runs;
+      const unsigned char* rec = p[0].pool.data() + o.pool_offset;
+      ketk::RunRecord h;
+      std::memcpy(&h, rec, sizeof h);
+      std::vector<ketk::RunOp> ops(h.n_ops);
+      std::memcpy(ops.data(), rec + sizeof h, ops.size() * sizeof(ketk::RunOp));
+      std::vector<ketk::RunSeg> segs(h.n_segs);
+      std::memcpy(segs.data(), rec + sizeof h + ops.size() * sizeof(ketk::RunOp),
+                  segs.size() * sizeof(ketk::RunSeg));
+      g_bank_conflicts += check_record(h, ops, segs, ketk::run_tile_bits(precision),
+                                       precision == 0 ? 3 : 4);
+    }
+    blocks += o.kind == ketplan::Kind::Block;
+  }
+  std::printf("ok learned %zu %d %d\n", p[0].ops.size(), runs, blocks);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc != 5 && argc != 6) {
-    std::fprintf(stderr, "usage: test_runs <records.bin> <n> <precision> <none|reference|features> [low bits]\n");
+    std::fprintf(stderr, "usage: test_runs <records.bin> <n> <precision> <none|reference|runs|features|learned> [low bits]\n");
     return 2;
   }
   std::ifstream f(argv[1], std::ios::binary);
@@ -446,8 +505,10 @@ int main(int argc, char** argv) {
     t.run_low_bits = std::atoi(argv[5]);
     ketk::set_tuning(precision, t);
   }
-  if (std::string(argv[4]) == "features") return features(g, n, precision);
-  const bool reference = std::string(argv[4]) == "reference";
-  return precision == 0 ? compare<double>(g, n, precision, reference)
-                        : compare<float>(g, n, precision, reference);
+  const std::string mode = argv[4];
+  if (mode == "features") return features(g, n, precision);
+  if (mode == "learned") return learned(g, n, precision);
+  const bool reference = mode == "reference", runs = mode == "runs";
+  return precision == 0 ? compare<double>(g, n, precision, reference, runs)
+                        : compare<float>(g, n, precision, reference, runs);
 }

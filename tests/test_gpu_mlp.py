@@ -56,37 +56,21 @@ def test_learned_plans_keep_the_state(seed):
 
 
 @pytest.mark.skipif(not os.path.exists(CM.MODEL_PATH), reason="no trained cost model committed")
-def test_trained_model_gain_over_greedy():
-    """What the trained network gains over the greedy planner on a few
-    held-out circuits of the training distribution (reported; the learned
-    planner must not lose more than timing noise)."""
+def test_trained_planner_against_fixed_forms():
+    """The trained planner on held-out circuits of the training distribution
+    (26-27 qubits here, complex64, resident states), against every fixed
+    form: whole-circuit wall time, each form warmed and timed as the median
+    of 3 cached repeats. It must be no slower than the best non-learned form
+    beyond the measured timing noise, and faster than the method without the
+    network (ket's Fuser as exact runs, its fallback) by more than it."""
     mlp = CM.load()
     states = CM.States(N.C64)
     try:
         circs = CM.config5_circuits(6, seed=999, lo=26, hi=27, depth=40)
-        res = CM.evaluate(mlp, circs, states)
+        res = CM.evaluate(mlp, circs, states, reps=3)
     finally:
         states.close()
-    print("learned vs greedy:", res)
-    assert res["speedup"] >= 0.9
-
-
-def test_learned_planner_beats_ket_fuser():
-    """The trained planner against ket's own fusion rule (KET_FUSE_REFERENCE,
-    the greedy Fuser of src/simulator.cpp:400-477, one pass per fused group as
-    ket runs it) on held-out circuits of the training distribution:
-    whole-circuit wall time, both in complex64 on resident states."""
-    mlp = CM.load()
-    states = CM.States(N.C64)
-    try:
-        circs = CM.config5_circuits(4, seed=4242, lo=26, hi=27, depth=40)
-        res = CM.evaluate(mlp, circs, states, ref_subset=len(circs))
-    finally:
-        states.close()
-    print("learned vs ket's Fuser:", res)
-    assert res["reference_fuser_circuits"] == len(circs)
-    assert res["speedup_vs_reference_fuser"] >= 2.0
-    # ket's Fuser as exact runs on LDS tiles is reported, not asserted: on
-    # these 26-27-qubit circuits in complex64 it is about as fast as fused
-    # blocks (a per-circuit choice between the two is not made yet)
-    assert res["speedup_vs_reference_fuser_runs"] > 0
+    print("learned planner vs fixed forms:", res)
+    tot, noise = res["total_ms"], res["noise"]
+    assert tot["learned"] <= tot[res["best_fixed_form"]] * (1 + noise)
+    assert tot["learned"] * (1 + noise) < tot[CM.FALLBACK]

@@ -349,6 +349,28 @@ def test_exact_runs_equal_per_gate_passes(cname, precision, restore_tuning):
 
 
 @pytest.mark.parametrize("precision", [N.C128, N.C64])
+@pytest.mark.parametrize("cname", sorted(CIRCUITS))
+def test_packed_runs(cname, precision):
+    """KET_FUSE_RUNS: per-gate passes with commuting gates pulled forward
+    into fewer exact runs; the rounding order changes, so the bar is that of
+    the fused modes (complex128 within 1e-12, complex64 1-F <= 1e-6), and
+    the plan has no more launches than KET_FUSE_EXACT's."""
+    c = CIRCUITS[cname]()
+    want = O.run(c)
+    got = gpu_run(c, N.FUSE_RUNS, precision)
+    if precision == N.C128:
+        assert np.abs(got - want).max() <= 1e-12
+        assert 1 - fid(got, want) <= 1e-12
+    else:
+        assert 1 - fid(got, want) <= 1e-6
+        assert np.abs(got - want).max() <= 1e-5
+    with N.StateVector(c.n_qubits, precision) as sv:
+        packed = sv.apply_circuit(c, N.FUSE_RUNS)["launches"]
+        exact = sv.apply_circuit(c, N.FUSE_EXACT)["launches"]
+    assert packed <= exact
+
+
+@pytest.mark.parametrize("precision", [N.C128, N.C64])
 @pytest.mark.parametrize("n", [3, 12, 15])
 def test_prob_prefix_is_kets_sequential_sum(n, precision):
     """ket_sv_prob_prefix equals the FP64 running sum measure() builds

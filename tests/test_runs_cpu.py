@@ -3,7 +3,8 @@ one pass per gate and with consecutive passes grouped into runs on LDS tiles
 are both executed on the CPU, the runs through their records exactly as
 k_run_tile walks them (segment layouts thread by thread, ops in their
 register form, through the LDS swizzle), and must give equal bits; for KET_FUSE_NONE against
-KET_FUSE_EXACT and for ket's Fuser with runs off against on."""
+KET_FUSE_EXACT and for ket's Fuser with runs off against on; KET_FUSE_RUNS
+(reordered passes) against one pass per gate within rounding."""
 import os
 import subprocess
 import tempfile
@@ -49,6 +50,31 @@ def test_runs_equal_per_gate_passes(name, precision, mode):
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", sorted(CIRCUITS))
+def test_packed_runs_equal_per_gate_passes_within_rounding(name, precision):
+    """KET_FUSE_RUNS reorders commuting passes into fewer runs: executed on
+    the CPU (runs through their records) it must equal one pass per gate in
+    program order within rounding (an illegal reordering changes amplitudes
+    by O(1)), with no more runs than KET_FUSE_EXACT makes."""
+    c = CIRCUITS[name]()
+    if c.n_qubits < (11 if precision == 0 else 12):
+        pytest.skip("below the tile size: no runs")
+    counts = {}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c.bin")
+        c.records().tofile(path)
+        for mode in ("none", "runs"):
+            r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), mode],
+                               capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+            ok, passes, runs, in_runs, conflicts = r.stdout.split()
+            assert ok == "ok"
+            counts[mode] = int(runs)
+    assert 1 <= counts["runs"] <= counts["none"]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
 @pytest.mark.parametrize("low", [3, 4, 7, 9])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_runs_equal_per_gate_passes_other_tile_shapes(precision, low):
@@ -66,8 +92,9 @@ def test_runs_equal_per_gate_passes_other_tile_shapes(precision, low):
 @pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
 @pytest.mark.parametrize("precision", [0, 1])
 def test_learned_planner_recording_features_match_full_packing(precision):
-    """The learned planner records candidate passes without building block
-    matrices; their cost-model features must equal the full packer's."""
+    """The learned planner predicts region plans whose block passes carry no
+    matrices; for every form it weighs, their cost-model features must equal
+    those of the full plan it then executes."""
     from ket_amd import costmodel as CM
     circs = CM.config5_circuits(4, seed=99, lo=20, hi=24) + [W.qft_u(20, 1), W.diag_mix(18, 10, 2),
                                                              W.hea(16, 8, 3)]
