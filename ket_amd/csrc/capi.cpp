@@ -147,7 +147,8 @@ int launch_op(ket_sv* sv, const ketplan::Op& o, const unsigned char* pool_dev) {
                              o.n_terms, pool_dev + o.pool_offset, ketplan::lowest_qubit(o));
       break;
     case ketplan::Kind::Run:
-      e = ketk::launch_run(sv->stream, sv->precision, sv->data, sv->n, pool_dev + o.pool_offset);
+      e = ketk::launch_run(sv->stream, sv->precision, sv->data, sv->n, pool_dev + o.pool_offset,
+                           o.run_contract);
       break;
   }
   if (e != hipSuccess) return hip_fail(e, "kernel launch");

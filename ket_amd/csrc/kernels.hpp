@@ -138,7 +138,10 @@ inline uint32_t run_slot(uint32_t t, int row_bits) {
 }
 inline int run_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 // n >= run_tile_bits(precision); rec_dev: the record in device memory.
-hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev);
+// contract: fused multiply-adds in the ops' arithmetic (the result then
+// equals the passes' within rounding only); false: bit for bit.
+hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
+                      bool contract = false);
 
 // Pool record of a fused block (see sv_block.hip). Matrix bit b of an
 // element index is qubit qubits[b]; offsets[e] = that element's index bits.

@@ -458,7 +458,7 @@ int run_low_bits(int precision) {
 
 }  // namespace
 
-void group_runs(Plan& plan, int n, int precision) {
+void group_runs(Plan& plan, int n, int precision, bool contract) {
   const int tb = ketk::run_tile_bits(precision);
   if (n < tb) return;
   const int low = run_low_bits(precision);
@@ -525,6 +525,7 @@ void group_runs(Plan& plan, int n, int precision) {
     o.run_pmask = P;
     o.run_ops = (int)ops.size();
     o.run_segs = (int)segs.size();
+    o.run_contract = contract;
     for (const auto& r : ops) {
       o.run_phase_ops += r.kind == ketk::RUN_PHASE;
       o.run_matrix_ops += r.kind == ketk::RUN_MATRIX;
@@ -716,7 +717,9 @@ bool default_decider(const BlockChoice& c) {
 // precision; 13 state size; 14 log2 of the pass's algorithmic bytes. A run's
 // width is its register layouts (segments), its gates the passes it
 // applies, its terms its phase passes, slot 8 its exact-matrix passes,
-// slot 10 its permutation passes and its highest qubit the tile's.
+// slot 10 its permutation passes, slot 11 is 0 when its arithmetic
+// contracts (fused multiply-adds), 1 when exact, and its highest qubit is
+// the tile's.
 void op_features(const Op& o, int n, int precision, float* f) {
   for (int i = 0; i < KET_MLP_INPUTS; ++i) f[i] = 0.f;
   const int L = line_bits(precision);
@@ -752,6 +755,7 @@ void op_features(const Op& o, int n, int precision, float* f) {
     f[8] = o.run_matrix_ops / 8.f;
     f[10] = o.run_perm_ops / 32.f;
   } else {
+    f[11] = (float)o.line_fraction;
     f[5] = width / 6.f;
     f[6] = std::min(o.n_gates, 128) / 32.f;
     f[7] = o.n_terms / 64.f;
@@ -759,7 +763,7 @@ void op_features(const Op& o, int n, int precision, float* f) {
     f[10] = __builtin_popcountll(qm & (bit(L) - 1)) / 4.f;
   }
   f[9] = qm ? (63 - __builtin_clzll(qm)) / 32.f : 0.f;
-  f[11] = (float)o.line_fraction;
+  if (o.kind == Kind::Run) f[11] = o.run_contract ? 0.f : 1.f;
   f[12] = (float)precision;
   f[13] = (n - 24) / 8.f;
   f[14] = (float)(std::log2(sweep * o.line_fraction) / 40.0);
@@ -1047,7 +1051,7 @@ bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precisi
     const std::vector<std::size_t> idx(parts[T.part].begin() + T.a, parts[T.part].begin() + T.b);
     Plan scratch;
     plan_region(gates, idx, n, precision, (Form)T.form, &scratch, true);
-    if (runs) group_runs(scratch, n, precision);
+    if (runs) group_runs(scratch, n, precision, true);
     T.rows.resize(scratch.ops.size() * KET_MLP_INPUTS);
     for (std::size_t i = 0; i < scratch.ops.size(); ++i)
       op_features(scratch.ops[i], n, precision, &T.rows[i * KET_MLP_INPUTS]);
@@ -1099,7 +1103,7 @@ bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precisi
       plan_region(gates, idx, n, precision, (Form)choice[h], plan, false);
     }
   }
-  if (runs) group_runs(*plan, n, precision);
+  if (runs) group_runs(*plan, n, precision, true);
   return true;
 }
 
@@ -1136,7 +1140,7 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
       else fuser_region(gates, run, n, precision, &plan);
     });
     if (fusion == KET_FUSE_RUNS || ketk::get_tuning(precision).exact_runs)
-      group_runs(plan, n, precision);
+      group_runs(plan, n, precision, fusion == KET_FUSE_RUNS);
     return KET_OK;
   }
   if (fusion == KET_FUSE_BLOCKS || fusion == KET_FUSE_BLOCKS_ALL) {

@@ -32,6 +32,7 @@ struct Op {
   int run_phase_ops = 0;                    // Run: phase passes among run_ops
   int run_matrix_ops = 0;                   // Run: exact-matrix passes among them
   int run_perm_ops = 0;                     // Run: permutation passes (X, SWAP, ...)
+  bool run_contract = false;                // Run: fused multiply-adds (see group_runs)
 };
 
 // A diagonal phase term: multiply by exp(i * angle) where all bits of mask
@@ -68,8 +69,10 @@ inline int default_block_k(int precision) {
 // applied on LDS tiles (ketk::launch_run) where that moves fewer bytes:
 // each run's tile positions hold every bit its passes move plus the lowest
 // others, at least ket_tuning.run_low_bits of them low contiguous bits.
-// Results are unchanged bit for bit. No-op below run_tile_bits qubits.
-void group_runs(Plan& plan, int n, int precision);
+// Results are unchanged bit for bit, unless contract is set: then the runs
+// use fused multiply-adds (equal within rounding; for the modes that are
+// not bit-exact anyway). No-op below run_tile_bits qubits.
+void group_runs(Plan& plan, int n, int precision, bool contract = false);
 
 // Lowest qubit of a MatrixExact or Block op.
 inline int lowest_qubit(const Op& o) {

@@ -896,12 +896,43 @@ hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uin
 // bits, so no amplitude moves between threads and no barrier is needed) and
 // writes them back to the slots it read; one barrier separates segments.
 
+// The complex arithmetic of a run's pair and matrix ops. FMA = false: each
+// product and sum rounded on its own, as in the per-gate kernels (runs that
+// must equal their passes bit for bit). FMA = true: fused multiply-adds
+// (runs of the modes whose results are only equal within rounding: 8
+// instead of 14 instructions per dense pair output, 4 instead of 8 per
+// matrix term). Phase products stay unfused in both: fused, their
+// mul -> fma chains measured slower on phase-heavy runs (complex128 QFT-30
+// as packed runs 149 -> 192 ms).
+template <class T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c);
+  else return __builtin_fmaf(a, b, c);
+}
+// m0 * a + m1 * b
+template <class T, bool FMA>
+__device__ __forceinline__ ketd::cplx<T> rdot(ketd::cplx<T> m0, ketd::cplx<T> a,
+                                              ketd::cplx<T> m1, ketd::cplx<T> b) {
+  if constexpr (FMA)
+    return ketd::mk<T>(fmad(m0.x, a.x, fmad(-m0.y, a.y, fmad(m1.x, b.x, -(m1.y * b.y)))),
+                       fmad(m0.x, a.y, fmad(m0.y, a.x, fmad(m1.x, b.y, m1.y * b.x))));
+  else return ketd::cadd<T>(ketd::cmul<T>(m0, a), ketd::cmul<T>(m1, b));
+}
+// acc + m * x
+template <class T, bool FMA>
+__device__ __forceinline__ ketd::cplx<T> rmac(ketd::cplx<T> acc, ketd::cplx<T> m,
+                                              ketd::cplx<T> x) {
+  if constexpr (FMA)
+    return ketd::mk<T>(fmad(m.x, x.x, fmad(-m.y, x.y, acc.x)), fmad(m.x, x.y, fmad(m.y, x.x, acc.y)));
+  else return ketd::cadd<T>(acc, ketd::cmul<T>(m, x));
+}
+
 // Pairs of one op whose register-index bits PR differ: (v[j], v[j ^ PR])
 // for the lower amplitudes j of rm (RunOp::reg_mask). LANE: lanes where
 // `keep` is set leave their amplitudes unchanged (a select per component:
 // no lane branches off, and a select of scalars, unlike one of whole
 // structs, keeps v in registers).
-template <class T, int RPT, int PR, bool LANE>
+template <class T, int RPT, int PR, bool LANE, bool FMA>
 __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uint32_t rm,
                                           const double* m, bool keep) {
   constexpr uint32_t kAll = [] {  // every lower amplitude: no register condition
@@ -919,8 +950,8 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
     for (int j = 0; j < RPT; ++j) {
       if ((j & PR) != 0) continue;  // compile time
       const ketd::cplx<T> a = v[j], b = v[j ^ PR];
-      ketd::cplx<T> x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
-      ketd::cplx<T> y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
+      ketd::cplx<T> x = rdot<T, FMA>(m00, a, m01, b);
+      ketd::cplx<T> y = rdot<T, FMA>(m10, a, m11, b);
       if constexpr (LANE) {
         x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
         y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
@@ -938,8 +969,8 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
     for (int j = 0; j < RPT; ++j) {
       if (!((rm >> j) & 1)) continue;  // uniform
       const ketd::cplx<T> a = v[j], b = v[j ^ PR];
-      ketd::cplx<T> x = ketd::cadd<T>(ketd::cmul<T>(m00, a), ketd::cmul<T>(m01, b));
-      ketd::cplx<T> y = ketd::cadd<T>(ketd::cmul<T>(m10, a), ketd::cmul<T>(m11, b));
+      ketd::cplx<T> x = rdot<T, FMA>(m00, a, m01, b);
+      ketd::cplx<T> y = rdot<T, FMA>(m10, a, m11, b);
       if constexpr (LANE) {
         x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
         y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
@@ -978,23 +1009,23 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
   }
 }
 
-template <class T, int RPT, bool LANE>
+template <class T, int RPT, bool LANE, bool FMA>
 __device__ __forceinline__ void reg_pair_op(ketd::cplx<T> (&v)[RPT], int kind, uint32_t part,
                                             uint32_t rm, const double* m, bool keep) {
   switch (part) {  // one or two register bits
-    case 1: reg_pairs<T, RPT, 1, LANE>(v, kind, rm, m, keep); break;
-    case 2: reg_pairs<T, RPT, 2, LANE>(v, kind, rm, m, keep); break;
-    case 4: reg_pairs<T, RPT, 4, LANE>(v, kind, rm, m, keep); break;
-    case 3: reg_pairs<T, RPT, 3, LANE>(v, kind, rm, m, keep); break;
-    case 5: reg_pairs<T, RPT, 5, LANE>(v, kind, rm, m, keep); break;
-    case 6: reg_pairs<T, RPT, 6, LANE>(v, kind, rm, m, keep); break;
+    case 1: reg_pairs<T, RPT, 1, LANE, FMA>(v, kind, rm, m, keep); break;
+    case 2: reg_pairs<T, RPT, 2, LANE, FMA>(v, kind, rm, m, keep); break;
+    case 4: reg_pairs<T, RPT, 4, LANE, FMA>(v, kind, rm, m, keep); break;
+    case 3: reg_pairs<T, RPT, 3, LANE, FMA>(v, kind, rm, m, keep); break;
+    case 5: reg_pairs<T, RPT, 5, LANE, FMA>(v, kind, rm, m, keep); break;
+    case 6: reg_pairs<T, RPT, 6, LANE, FMA>(v, kind, rm, m, keep); break;
     default:
       if constexpr (RPT == 16) {
         switch (part) {
-          case 8: reg_pairs<T, RPT, 8, LANE>(v, kind, rm, m, keep); break;
-          case 9: reg_pairs<T, RPT, 9, LANE>(v, kind, rm, m, keep); break;
-          case 10: reg_pairs<T, RPT, 10, LANE>(v, kind, rm, m, keep); break;
-          case 12: reg_pairs<T, RPT, 12, LANE>(v, kind, rm, m, keep); break;
+          case 8: reg_pairs<T, RPT, 8, LANE, FMA>(v, kind, rm, m, keep); break;
+          case 9: reg_pairs<T, RPT, 9, LANE, FMA>(v, kind, rm, m, keep); break;
+          case 10: reg_pairs<T, RPT, 10, LANE, FMA>(v, kind, rm, m, keep); break;
+          case 12: reg_pairs<T, RPT, 12, LANE, FMA>(v, kind, rm, m, keep); break;
           default: break;
         }
       }
@@ -1005,7 +1036,7 @@ __device__ __forceinline__ void reg_pair_op(ketd::cplx<T> (&v)[RPT], int kind, u
 // An exact-matrix op of width K on register bits 0..K-1: per group, acc = 0;
 // acc += M[r][c] * in[c] for c ascending (the matrix converted to T element
 // by element, as k_matrix_exact does).
-template <class T, int RPT, int K>
+template <class T, int RPT, int K, bool FMA>
 __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double2* mat) {
   constexpr int D = 1 << K;
 #pragma unroll
@@ -1021,9 +1052,7 @@ __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double
       ketd::cplx<T> acc = ketd::mk<T>(T(0), T(0));
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        acc = ketd::cadd<T>(acc, ketd::cmul<T>(ketd::mk<T>((T)mat[r * D + c].x,
-                                                           (T)mat[r * D + c].y),
-                                               in[c]));
+        acc = rmac<T, FMA>(acc, ketd::mk<T>((T)mat[r * D + c].x, (T)mat[r * D + c].y), in[c]);
       v[g * D + r] = acc;
     }
   }
@@ -1086,7 +1115,7 @@ __device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunOp*
 
 // One pair or exact-matrix op on the thread's registers. Every branch here
 // is uniform over the workgroup; the lane condition (reg_lane) is a select.
-template <class T, int RPT>
+template <class T, int RPT, bool FMA>
 __device__ __forceinline__ void reg_op(ketd::cplx<T> (&v)[RPT], const RunOp* opp, uint32_t base,
                                        uint64_t fixed, const unsigned char* rec) {
   const int kind = opp->kind;
@@ -1097,16 +1126,16 @@ __device__ __forceinline__ void reg_op(ketd::cplx<T> (&v)[RPT], const RunOp* opp
   if (kind == RUN_MATRIX) {
     const auto* mat = reinterpret_cast<const double2*>(rec + opp->mat_offset);
     const int k = opp->npos;
-    if (k == 1) reg_matrix<T, RPT, 1>(v, mat);
-    else if (k == 2) reg_matrix<T, RPT, 2>(v, mat);
-    else reg_matrix<T, RPT, 3>(v, mat);
+    if (k == 1) reg_matrix<T, RPT, 1, FMA>(v, mat);
+    else if (k == 2) reg_matrix<T, RPT, 2, FMA>(v, mat);
+    else reg_matrix<T, RPT, 3, FMA>(v, mat);
     return;
   }
   const uint32_t part = opp->reg_part, rm = opp->reg_mask;
   if (lane == 0)
-    reg_pair_op<T, RPT, false>(v, kind, part, rm, opp->m, false);
+    reg_pair_op<T, RPT, false, FMA>(v, kind, part, rm, opp->m, false);
   else
-    reg_pair_op<T, RPT, true>(v, kind, part, rm, opp->m, keep);
+    reg_pair_op<T, RPT, true, FMA>(v, kind, part, rm, opp->m, keep);
 }
 
 // Moves a run's tile between HBM and LDS in index order (16-byte accesses,
@@ -1147,11 +1176,14 @@ struct RunIO {
       if ((j >> b) & 1) a ^= lb[b];
     return a;
   }
-  __device__ __forceinline__ void load(const ketd::cplx<T>* __restrict__ s, uint64_t fixed,
-                                       ketd::cplx<T>* L) const {
-    V v[S];
+  // The thread's share of a tile, HBM -> registers (issued, not awaited)
+  // and registers -> LDS.
+  __device__ __forceinline__ void gload(const ketd::cplx<T>* __restrict__ s, uint64_t fixed,
+                                        V (&v)[S]) const {
 #pragma unroll
     for (int j = 0; j < S; ++j) v[j] = load_nt(reinterpret_cast<const V*>(s + (fixed | gaddr(j))));
+  }
+  __device__ __forceinline__ void lput(ketd::cplx<T>* L, const V (&v)[S]) const {
 #pragma unroll
     for (int j = 0; j < S; ++j) IO::put(L, laddr(j), v[j]);
   }
@@ -1165,7 +1197,7 @@ struct RunIO {
 
 // One workgroup per tile (grid-stride): load the tile, apply each segment's
 // ops in registers, store it back.
-template <class T, int TB>
+template <class T, int TB, bool FMA>
 __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict__ s,
                                                         uint64_t n_tiles,
                                                         const unsigned char* __restrict__ rec) {
@@ -1179,20 +1211,30 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
   io.init(h, threadIdx.x);
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     const uint64_t fixed = deposit_outside(tile, h.pmask);
-    io.load(s, fixed, L);
+    {
+      typename RunIO<T, TB>::V v[RunIO<T, TB>::S];
+      io.gload(s, fixed, v);
+      io.lput(L, v);
+    }
     for (int sg = 0; sg < h.n_segs; ++sg) {
       __syncthreads();
       const RunSeg& S = segs[sg];
+      // The 16-bit tables read as 32-bit words: scalar loads (a sub-dword
+      // load would be a vector one, and its wait would also wait for the
+      // next tile's loads).
+      const auto* tbit = reinterpret_cast<const uint32_t*>(S.tbit);
+      const auto* tslot = reinterpret_cast<const uint32_t*>(S.tslot);
+      const auto* rslot = reinterpret_cast<const uint32_t*>(S.rslot);
       uint32_t base = 0, slot = 0;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {  // branch-free, so the tables load as scalars
         const uint32_t on = 0u - ((threadIdx.x >> i) & 1u);
-        base |= (uint32_t)S.tbit[i] & on;
-        slot ^= (uint32_t)S.tslot[i] & on;
+        base |= (tbit[i >> 1] >> (16 * (i & 1))) & 0xffffu & on;
+        slot ^= (tslot[i >> 1] >> (16 * (i & 1))) & 0xffffu & on;
       }
       ketd::cplx<T> v[RPT];
 #pragma unroll
-      for (int j = 0; j < RPT; ++j) v[j] = L[slot ^ S.rslot[j]];
+      for (int j = 0; j < RPT; ++j) v[j] = L[slot ^ ((rslot[j >> 1] >> (16 * (j & 1))) & 0xffffu)];
       const int end = S.first_op + S.n_ops;
       for (int o = S.first_op; o < end;) {
         const RunOp* op = ops + o;
@@ -1200,12 +1242,14 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
           reg_phases<T, RPT>(v, op, op->stretch, base, fixed);
           o += op->stretch;
         } else {
-          reg_op<T, RPT>(v, op, base, fixed, rec);
+          reg_op<T, RPT, FMA>(v, op, base, fixed, rec);
           ++o;
         }
       }
+      // the slots again (recomputed: not held in registers across the ops)
+      __asm__ volatile("" ::: "memory");
 #pragma unroll
-      for (int j = 0; j < RPT; ++j) L[slot ^ S.rslot[j]] = v[j];
+      for (int j = 0; j < RPT; ++j) L[slot ^ ((rslot[j >> 1] >> (16 * (j & 1))) & 0xffffu)] = v[j];
     }
     __syncthreads();
     io.store(s, fixed, L);
@@ -1280,7 +1324,8 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
             : block_vec<float, false>(st, s, groups, k, rec);
 }
 
-hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev) {
+hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
+                      bool contract) {
   const int tb = run_tile_bits(precision);
   if (n < tb) return hipErrorInvalidValue;
   const uint64_t n_tiles = uint64_t{1} << (n - tb);
@@ -1289,10 +1334,12 @@ hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void*
   const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
   if (g > cap) g = cap;
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
-  if (precision == 0)
-    k_run_tile<double, 11><<<(unsigned)g, kThreads, 0, st>>>(static_cast<double2*>(s), n_tiles, rec);
-  else
-    k_run_tile<float, 12><<<(unsigned)g, kThreads, 0, st>>>(static_cast<float2*>(s), n_tiles, rec);
+  auto* d = static_cast<double2*>(s);
+  auto* f = static_cast<float2*>(s);
+  if (precision == 0 && contract) k_run_tile<double, 11, true><<<(unsigned)g, kThreads, 0, st>>>(d, n_tiles, rec);
+  else if (precision == 0) k_run_tile<double, 11, false><<<(unsigned)g, kThreads, 0, st>>>(d, n_tiles, rec);
+  else if (contract) k_run_tile<float, 12, true><<<(unsigned)g, kThreads, 0, st>>>(f, n_tiles, rec);
+  else k_run_tile<float, 12, false><<<(unsigned)g, kThreads, 0, st>>>(f, n_tiles, rec);
   return hipGetLastError();
 }
 
