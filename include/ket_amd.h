@@ -73,9 +73,11 @@ typedef enum ket_fusion {
   KET_FUSE_BLOCKS = 2,    /* k-qubit packer: dense 2^k blocks (k = block_k) */
   KET_FUSE_LEARNED = 3,   /* per region of the circuit, the form (runs,
                              ket's Fuser, blocks of width 3, 4 or 5) a
-                             trained cost model predicts fastest; without a
-                             model it is KET_FUSE_REFERENCE. block_k is not
-                             used (0 or 2..6 accepted) */
+                             trained cost model predicts fastest, the run
+                             packer unless another form is predicted 8%
+                             faster; without a model it is
+                             KET_FUSE_REFERENCE. block_k is not used (0 or
+                             2..6 accepted) */
   KET_FUSE_BLOCKS_ALL = 4, /* the packer with every group emitted as a block
                              (cost-model training data, batched sweeps) */
   KET_FUSE_EXACT = 5,     /* KET_FUSE_NONE's per-gate arithmetic with

@@ -70,8 +70,11 @@ hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
 // hold every bit an op moves; local bit j of a tile index is the j-th-lowest
 // bit of P.
 //
-// The ops are applied in registers: 256 threads each hold RPT = 2^(TB-8)
-// amplitudes of the tile. A segment is a stretch of consecutive ops whose
+// The ops are applied in registers: 2^(TB-RB) threads each hold RPT = 2^RB
+// amplitudes of the tile (RB = run_reg_bits: 8 complex128 or 16 complex64
+// amplitudes per thread, 256 threads; with 16 and 32, 128 threads, the
+// kernel measured 20-40% slower: a wave's op sequence is latency-bound and
+// fewer waves hide less of it). A segment is a stretch of consecutive ops whose
 // moved bits (pair partners, matrix bits) are all register bits of one
 // layout: register bit r of a thread's amplitude index is tile bit rpos[r],
 // thread bit i is tile bit tpos[i]. Within a segment no amplitude moves
@@ -82,12 +85,18 @@ hipError_t launch_convert(hipStream_t st, int dst_precision, void* dst,
 // log2(128 / amplitude bytes): a fixed XOR swizzle under which lanes whose
 // tile bits have distinct residues mod RB reach distinct slots of a
 // 128-byte LDS row. A record is a RunRecord, n_ops RunOps, n_segs RunSegs,
-// then the 4^k complex128 matrices of its exact-matrix ops.
-enum RunKind { RUN_DENSE = 0, RUN_CH = 1, RUN_SWAP = 2, RUN_PHASE = 3, RUN_MATRIX = 4 };
+// the RunPhase table (the entries of each phase stretch in op order, padded
+// with zero entries to a multiple of kRunPhaseGroup), then the 4^k
+// complex128 matrices of its exact-matrix ops.
+// RUN_DENSE_REAL: a dense 2x2 whose entries all have zero imaginary parts
+// (H, RY, real rotations), applied with real products (equal in value).
+enum RunKind {
+  RUN_DENSE = 0, RUN_CH = 1, RUN_SWAP = 2, RUN_PHASE = 3, RUN_MATRIX = 4, RUN_DENSE_REAL = 5
+};
 struct RunOp {
   // read by k_run_tile: the op in its segment's register layout (matrix
-  // ops: matrix bit b is register bit b, so the reg_* fields stay 0). The
-  // first 32 bytes are all a phase op needs, so they load as one.
+  // ops: matrix bit b is register bit b, so the reg_* fields stay 0; phase
+  // ops are read from their RunPhase).
   uint64_t out_mask;   // bits outside P that must all be set (outside controls)
   uint32_t reg_lane;   // tile bits held by thread bits that must be set
   uint32_t reg_mask;   // registers j the op applies to: phases, j holding
@@ -101,7 +110,9 @@ struct RunOp {
   uint32_t reg_lo;     // their values in the pair's lower amplitude
   uint32_t stretch;    // phase ops: consecutive phase ops of the segment from
                        // this one on (this one included)
-  uint32_t mat_offset; // matrix ops: bytes from the record start
+  uint32_t mat_offset; // matrix ops: bytes from the record start; phase
+                       // ops: index of the op's RunPhase (a stretch's
+                       // entries are consecutive)
   int32_t pad;
   // the op's local form (planner, CPU interpreter)
   int32_t pos[4];      // local positions, ascending (zero bits inserted there)
@@ -109,7 +120,8 @@ struct RunOp {
   uint32_t set;        // local bits set in the lower index of a pair / phase
   uint32_t partner;    // local bits a pair's two indices differ in
 };
-constexpr int kRunMaxRegs = 16;  // amplitudes per thread (complex64: 2^(12-8))
+constexpr int kRunMaxRegs = 32;        // amplitudes per thread (complex64: 2^5)
+constexpr int kRunMaxThreadBits = 8;  // thread bits of a run workgroup
 struct RunSeg {
   int32_t first_op;
   int32_t n_ops;
@@ -117,17 +129,28 @@ struct RunSeg {
   int32_t pad;
   uint16_t roff[kRunMaxRegs];     // register j: its tile-index bits
   uint16_t rslot[kRunMaxRegs];    // ... and their slot bits
-  uint16_t tbit[8];               // thread bit i: its tile-index bit
-  uint16_t tslot[8];              // ... and its slot bits
+  uint16_t tbit[kRunMaxThreadBits];   // thread bit i: its tile-index bit
+  uint16_t tslot[kRunMaxThreadBits];  // ... and its slot bits
 };
 struct RunRecord {
   uint64_t pmask;
   int32_t n_ops;
   int32_t n_segs;
   uint16_t sbit[12];  // slot bits of tile bit b
-  int32_t pad[2];
+  uint32_t phase_offset;  // bytes from the record start to the RunPhase table
+  int32_t pad;
 };
-static_assert(sizeof(RunOp) == 152 && sizeof(RunSeg) == 112 && sizeof(RunRecord) == 48,
+// A phase op as k_run_tile reads it: 32 bytes, so a group of four loads as
+// two scalar 64-byte loads. An entry with reg_mask 0 is a no-op.
+struct RunPhase {
+  uint64_t out_mask;  // = the op's RunOp::out_mask
+  uint32_t reg_lane;  // = RunOp::reg_lane
+  uint32_t reg_mask;  // = RunOp::reg_mask (registers j holding reg_cond)
+  double re, im;      // the phase
+};
+constexpr int kRunPhaseGroup = 4;
+static_assert(sizeof(RunOp) == 152 && sizeof(RunSeg) == 176 && sizeof(RunRecord) == 48 &&
+                  sizeof(RunPhase) == 32,
               "run record layout");
 // Slot bits of tile index t under the runs' LDS swizzle.
 inline uint32_t run_slot(uint32_t t, int row_bits) {
@@ -137,6 +160,10 @@ inline uint32_t run_slot(uint32_t t, int row_bits) {
   return (t & ~((1u << row_bits) - 1)) | x;
 }
 inline int run_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
+// Register bits of a run thread's amplitude index (RPT = 2^RB amplitudes per
+// thread); the workgroup has 2^(TB - RB) threads.
+inline int run_reg_bits(int precision) { return precision == 0 ? 3 : 4; }
+inline int run_thread_bits(int precision) { return run_tile_bits(precision) - run_reg_bits(precision); }
 // n >= run_tile_bits(precision); rec_dev: the record in device memory.
 // contract: fused multiply-adds in the ops' arithmetic (the result then
 // equals the passes' within rounding only); false: bit for bit.

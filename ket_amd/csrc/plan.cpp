@@ -293,6 +293,9 @@ bool local_op(const Plan& plan, const Op& o, uint64_t P, ketk::RunOp* r,
             : o.pair_op == ketk::OP_CH   ? ketk::RUN_CH
             : o.pair_op == ketk::OP_SWAP ? ketk::RUN_SWAP
                                          : ketk::RUN_DENSE;
+  if (r->kind == ketk::RUN_DENSE && o.m[1] == 0.0 && o.m[3] == 0.0 && o.m[5] == 0.0 &&
+      o.m[7] == 0.0)
+    r->kind = ketk::RUN_DENSE_REAL;
   for (int i = 0; i < 8; ++i) r->m[i] = o.m[i];
   int np = 0;
   for (int i = 0; i < o.spec.npos; ++i) {
@@ -342,12 +345,13 @@ uint32_t to_regs(uint32_t x, const int* rpos, int nr) {
 // share one bit order (matrix bit b = register bit b). Its register bits are
 // those moved bits plus the tile bits its ops test most often (a test on a
 // register bit skips whole registers); of the other bits the most tested
-// become the two wave bits (thread bits 6, 7: skipped per wave), and the
+// become the wave bits (thread bits 6 and up: skipped per wave), and the
 // thread bits 0..rb-1, whose lanes share LDS banks in one access, take the
 // least tested bits of distinct residues mod rb where they exist (distinct
-// rows of the swizzle).
-std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, int row_bits) {
-  const int nr = tb - 8;
+// rows of the swizzle). nr = register bits, tb - nr thread bits.
+std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, int nr,
+                                       int row_bits) {
+  const int n_thread_bits = tb - nr;
   std::vector<ketk::RunSeg> segs;
   std::size_t i = 0;
   while (i < ops.size()) {
@@ -369,7 +373,7 @@ std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, in
     int use[16] = {};
     for (std::size_t t = i; t < j; ++t)
       for (uint32_t c = run_cond(ops[t]); c; c &= c - 1) ++use[__builtin_ctz(c)];
-    int rpos[4], n_r = 0;
+    int rpos[5], n_r = 0;
     uint32_t rmask = 0;
     auto take = [&](int b) {
       if (n_r < nr && !((rmask >> b) & 1)) {
@@ -390,7 +394,7 @@ std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, in
     for (int b = 0; b < tb; ++b)
       if (!((rmask >> b) & 1)) rest.push_back(b);
     std::stable_sort(rest.begin(), rest.end(), [&](int a, int b) { return use[a] < use[b]; });
-    int tpos[8];
+    int tpos[ketk::kRunMaxThreadBits];
     bool used[16] = {};
     bool res[8] = {};
     int nt = 0;
@@ -406,7 +410,7 @@ std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, in
         tpos[nt++] = b;
       }
     for (int b : rest)
-      if (!used[b]) tpos[nt++] = b;  // ascending use: the last two are the wave bits
+      if (!used[b]) tpos[nt++] = b;  // ascending use: the last ones are the wave bits
     ketk::RunSeg sg{};
     sg.first_op = (int32_t)i;
     sg.n_ops = (int32_t)(j - i);
@@ -418,7 +422,7 @@ std::vector<ketk::RunSeg> run_segments(std::vector<ketk::RunOp>& ops, int tb, in
       sg.roff[r] = (uint16_t)t;
       sg.rslot[r] = (uint16_t)ketk::run_slot(t, row_bits);
     }
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < n_thread_bits; ++b) {
       sg.tbit[b] = (uint16_t)(1u << tpos[b]);
       sg.tslot[b] = (uint16_t)ketk::run_slot(1u << tpos[b], row_bits);
     }
@@ -500,9 +504,22 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
       continue;
     }
     const int row_bits = line_bits(precision);
-    const std::vector<ketk::RunSeg> segs = run_segments(ops, tb, row_bits);
-    const std::size_t head = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp) +
-                             segs.size() * sizeof(ketk::RunSeg);
+    const std::vector<ketk::RunSeg> segs =
+        run_segments(ops, tb, ketk::run_reg_bits(precision), row_bits);
+    // the phase table: each stretch's entries, padded to whole groups
+    std::vector<ketk::RunPhase> phases;
+    for (std::size_t t = 0; t < ops.size(); ++t) {
+      ketk::RunOp& r = ops[t];
+      if (r.kind != ketk::RUN_PHASE) continue;
+      r.mat_offset = (uint32_t)phases.size();
+      phases.push_back({r.out_mask, r.reg_lane, r.reg_mask, r.m[0], r.m[1]});
+      const bool last = r.stretch == 1;  // the stretch ends here
+      if (last)
+        while (phases.size() % ketk::kRunPhaseGroup) phases.push_back(ketk::RunPhase{});
+    }
+    const std::size_t phase_at = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp) +
+                                 segs.size() * sizeof(ketk::RunSeg);
+    const std::size_t head = phase_at + phases.size() * sizeof(ketk::RunPhase);
     for (auto& r : ops)
       if (r.kind == ketk::RUN_MATRIX) r.mat_offset += (uint32_t)head;
     const std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
@@ -511,12 +528,15 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
     h.pmask = P;
     h.n_ops = (int)ops.size();
     h.n_segs = (int)segs.size();
+    h.phase_offset = (uint32_t)phase_at;
     for (int b = 0; b < tb; ++b) h.sbit[b] = (uint16_t)ketk::run_slot(1u << b, row_bits);
     unsigned char* dst = plan.pool.data() + off;
     std::memcpy(dst, &h, sizeof h);
     std::memcpy(dst + sizeof h, ops.data(), ops.size() * sizeof(ketk::RunOp));
     std::memcpy(dst + sizeof h + ops.size() * sizeof(ketk::RunOp), segs.data(),
                 segs.size() * sizeof(ketk::RunSeg));
+    if (!phases.empty())
+      std::memcpy(dst + phase_at, phases.data(), phases.size() * sizeof(ketk::RunPhase));
     if (!mats.empty()) std::memcpy(dst + head, mats.data(), mats.size());
     Op o;
     o.kind = Kind::Run;
@@ -1016,13 +1036,17 @@ namespace {
 // planned in every form (the run packer, ket's Fuser, the block packer at
 // widths 3, 4 and 5 with its bytes rule, each followed by the grouping into
 // exact runs as the final plan does it), the cost model predicts the time
-// of all of their passes in one GPU inference, and a region keeps its
-// cheapest form unless its two halves, each planned at its best, cost less.
+// of all of their passes in one GPU inference, and a region keeps the run
+// packer unless another form, or its two halves each planned at their best,
+// are predicted cheaper by more than kSwitchMargin.
 // Candidate block passes are planned without matrices (features only). The
 // chosen regions are then planned in full, in order, and grouped into runs
 // once. False when no model is set.
 constexpr std::size_t kMinRegion = 48;
 constexpr int kMaxLevel = 3;
+// A prediction must beat the run packer's by this fraction before a region
+// leaves it (the model's median error on whole circuits is ~6%).
+constexpr double kSwitchMargin = 0.08;
 
 bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precision, Plan* plan) {
   if (!planner_model_set()) return false;
@@ -1075,12 +1099,19 @@ bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precisi
     std::vector<double> best(nodes);
     std::vector<int> choice(nodes);  // a form, or -1: split in halves
     for (std::size_t h = nodes; h-- > 0;) {
-      int f = 0;
-      for (int x = 1; x < kForms; ++x)
-        if (cost[p][h][x] < cost[p][h][f]) f = x;
+      // The run packer unless another form (or a split) is predicted
+      // cheaper by more than the model's error.
+      int f = (int)Form::Runs;
+      double keep = cost[p][h][f] * (1.0 - kSwitchMargin);
+      for (int x = 0; x < kForms; ++x)
+        if (x != (int)Form::Runs && cost[p][h][x] < keep) {
+          f = x;
+          keep = cost[p][h][x];
+        }
       best[h] = cost[p][h][f];
       choice[h] = f;
-      if (2 * h + 2 < nodes && best[2 * h + 1] + best[2 * h + 2] < best[h]) {
+      if (2 * h + 2 < nodes &&
+          best[2 * h + 1] + best[2 * h + 2] < best[h] * (1.0 - kSwitchMargin)) {
         best[h] = best[2 * h + 1] + best[2 * h + 2];
         choice[h] = -1;
       }

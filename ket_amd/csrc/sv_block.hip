@@ -890,8 +890,9 @@ hipError_t block_vec_batched(hipStream_t st, void* s, uint64_t state_stride, uin
 // Each op repeats its per-gate kernel's arithmetic (k_pair, k_phase,
 // k_matrix_exact in sv_gates.hip) on the tile's amplitudes, in plan order,
 // so every amplitude sees the same operations on the same values as with one
-// pass per op. The tile is loaded into LDS in index order; then each segment
-// of ops (RunSeg) reads the thread's RPT amplitudes of its layout into
+// pass per op. A workgroup of NT = 2^(TB - RB) threads holds a tile of 2^TB
+// amplitudes. The tile is loaded into LDS in index order; then each segment
+// of ops (RunSeg) reads the thread's RPT = 2^RB amplitudes of its layout into
 // registers, applies its ops there (partners and matrix bits are register
 // bits, so no amplitude moves between threads and no barrier is needed) and
 // writes them back to the slots it read; one barrier separates segments.
@@ -927,6 +928,53 @@ __device__ __forceinline__ ketd::cplx<T> rmac(ketd::cplx<T> acc, ketd::cplx<T> m
   else return ketd::cadd<T>(acc, ketd::cmul<T>(m, x));
 }
 
+// The outputs of a dense 2x2 on (a, b): x = m00 a + m01 b, y = m10 a + m11 b
+// with ket's complex products. REAL: every entry's imaginary part is zero;
+// ket's product (m, 0)(a.x, a.y) = (m a.x - 0 a.y, m a.y + 0 a.x) then equals
+// (m a.x, m a.y) in value (the dropped terms are signed zeros), so x and y
+// equal ket's value for value with 12 operations instead of 28.
+template <class T, bool FMA, bool REAL>
+__device__ __forceinline__ void dense_out(const ketd::cplx<T> (&M)[4], ketd::cplx<T> a,
+                                          ketd::cplx<T> b, ketd::cplx<T>& x, ketd::cplx<T>& y) {
+  if constexpr (REAL) {
+    if constexpr (FMA) {
+      x = ketd::mk<T>(fmad(M[0].x, a.x, M[1].x * b.x), fmad(M[0].x, a.y, M[1].x * b.y));
+      y = ketd::mk<T>(fmad(M[2].x, a.x, M[3].x * b.x), fmad(M[2].x, a.y, M[3].x * b.y));
+    } else {
+      x = ketd::mk<T>(M[0].x * a.x + M[1].x * b.x, M[0].x * a.y + M[1].x * b.y);
+      y = ketd::mk<T>(M[2].x * a.x + M[3].x * b.x, M[2].x * a.y + M[3].x * b.y);
+    }
+  } else {
+    x = rdot<T, FMA>(M[0], a, M[1], b);
+    y = rdot<T, FMA>(M[2], a, M[3], b);
+  }
+}
+
+// A dense op on the pairs (v[j], v[j ^ PR]) of the lower amplitudes j in rm
+// (all = every lower amplitude: one straight block, no register tests).
+template <class T, int RPT, int PR, bool LANE, bool FMA, bool REAL>
+__device__ __forceinline__ void dense_regs(ketd::cplx<T> (&v)[RPT], uint32_t rm, uint32_t all,
+                                           const double* m, bool keep) {
+  const ketd::cplx<T> M[4] = {ketd::mk<T>((T)m[0], (T)m[1]), ketd::mk<T>((T)m[2], (T)m[3]),
+                              ketd::mk<T>((T)m[4], (T)m[5]), ketd::mk<T>((T)m[6], (T)m[7])};
+  const bool straight = rm == all;  // uniform
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) {
+    if ((j & PR) != 0) continue;                // compile time
+    if (!straight && !((rm >> j) & 1)) continue;  // uniform
+    const ketd::cplx<T> a = v[j], b = v[j ^ PR];
+    ketd::cplx<T> x, y;
+    dense_out<T, FMA, REAL>(M, a, b, x, y);
+    if constexpr (LANE) {
+      x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
+      y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
+    }
+    v[j] = x;
+    v[j ^ PR] = y;
+    __builtin_amdgcn_sched_barrier(0);  // one pair in flight
+  }
+}
+
 // Pairs of one op whose register-index bits PR differ: (v[j], v[j ^ PR])
 // for the lower amplitudes j of rm (RunOp::reg_mask). LANE: lanes where
 // `keep` is set leave their amplitudes unchanged (a select per component:
@@ -941,43 +989,10 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
       if ((j & PR) == 0) x |= 1u << j;
     return x;
   }();
-  if (kind == RUN_DENSE && rm == kAll) {  // straight block
-    const ketd::cplx<T> m00 = ketd::mk<T>((T)m[0], (T)m[1]);
-    const ketd::cplx<T> m01 = ketd::mk<T>((T)m[2], (T)m[3]);
-    const ketd::cplx<T> m10 = ketd::mk<T>((T)m[4], (T)m[5]);
-    const ketd::cplx<T> m11 = ketd::mk<T>((T)m[6], (T)m[7]);
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-      if ((j & PR) != 0) continue;  // compile time
-      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
-      ketd::cplx<T> x = rdot<T, FMA>(m00, a, m01, b);
-      ketd::cplx<T> y = rdot<T, FMA>(m10, a, m11, b);
-      if constexpr (LANE) {
-        x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
-        y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
-      }
-      v[j] = x;
-      v[j ^ PR] = y;
-      if (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);  // one pair in flight
-    }
+  if (kind == RUN_DENSE_REAL) {
+    dense_regs<T, RPT, PR, LANE, FMA, true>(v, rm, kAll, m, keep);
   } else if (kind == RUN_DENSE) {
-    const ketd::cplx<T> m00 = ketd::mk<T>((T)m[0], (T)m[1]);
-    const ketd::cplx<T> m01 = ketd::mk<T>((T)m[2], (T)m[3]);
-    const ketd::cplx<T> m10 = ketd::mk<T>((T)m[4], (T)m[5]);
-    const ketd::cplx<T> m11 = ketd::mk<T>((T)m[6], (T)m[7]);
-#pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-      if (!((rm >> j) & 1)) continue;  // uniform
-      const ketd::cplx<T> a = v[j], b = v[j ^ PR];
-      ketd::cplx<T> x = rdot<T, FMA>(m00, a, m01, b);
-      ketd::cplx<T> y = rdot<T, FMA>(m10, a, m11, b);
-      if constexpr (LANE) {
-        x = ketd::mk<T>(keep ? a.x : x.x, keep ? a.y : x.y);
-        y = ketd::mk<T>(keep ? b.x : y.x, keep ? b.y : y.y);
-      }
-      v[j] = x;
-      v[j ^ PR] = y;
-    }
+    dense_regs<T, RPT, PR, LANE, FMA, false>(v, rm, kAll, m, keep);
   } else if (kind == RUN_CH) {
     const T r = (T)m[0];
 #pragma unroll
@@ -992,6 +1007,15 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
       }
       v[j] = x;
       v[j ^ PR] = y;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else if (!LANE && rm == kAll) {  // a swap of whole registers: renaming
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      if ((j & PR) != 0) continue;  // compile time
+      const ketd::cplx<T> a = v[j];
+      v[j] = v[j ^ PR];
+      v[j ^ PR] = a;
     }
   } else {
 #pragma unroll
@@ -1009,27 +1033,19 @@ __device__ __forceinline__ void reg_pairs(ketd::cplx<T> (&v)[RPT], int kind, uin
   }
 }
 
-template <class T, int RPT, bool LANE, bool FMA>
+// reg_pairs for a wave-uniform partner mask `part` of one or two register
+// bits: a chain of uniform tests over the masks PR, PR + 1, ... < RPT.
+template <class T, int RPT, bool LANE, bool FMA, int PR = 1>
 __device__ __forceinline__ void reg_pair_op(ketd::cplx<T> (&v)[RPT], int kind, uint32_t part,
                                             uint32_t rm, const double* m, bool keep) {
-  switch (part) {  // one or two register bits
-    case 1: reg_pairs<T, RPT, 1, LANE, FMA>(v, kind, rm, m, keep); break;
-    case 2: reg_pairs<T, RPT, 2, LANE, FMA>(v, kind, rm, m, keep); break;
-    case 4: reg_pairs<T, RPT, 4, LANE, FMA>(v, kind, rm, m, keep); break;
-    case 3: reg_pairs<T, RPT, 3, LANE, FMA>(v, kind, rm, m, keep); break;
-    case 5: reg_pairs<T, RPT, 5, LANE, FMA>(v, kind, rm, m, keep); break;
-    case 6: reg_pairs<T, RPT, 6, LANE, FMA>(v, kind, rm, m, keep); break;
-    default:
-      if constexpr (RPT == 16) {
-        switch (part) {
-          case 8: reg_pairs<T, RPT, 8, LANE, FMA>(v, kind, rm, m, keep); break;
-          case 9: reg_pairs<T, RPT, 9, LANE, FMA>(v, kind, rm, m, keep); break;
-          case 10: reg_pairs<T, RPT, 10, LANE, FMA>(v, kind, rm, m, keep); break;
-          case 12: reg_pairs<T, RPT, 12, LANE, FMA>(v, kind, rm, m, keep); break;
-          default: break;
-        }
+  if constexpr (PR < RPT) {
+    if constexpr (__builtin_popcount(PR) <= 2) {
+      if (part == (uint32_t)PR) {
+        reg_pairs<T, RPT, PR, LANE, FMA>(v, kind, rm, m, keep);
+        return;
       }
-      break;
+    }
+    reg_pair_op<T, RPT, LANE, FMA, PR + 1>(v, kind, part, rm, m, keep);
   }
 }
 
@@ -1058,13 +1074,6 @@ __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double
   }
 }
 
-// A stretch of consecutive phase ops on the thread's registers, one path
-// per op: whole tiles missing an op's outside bits and registers missing
-// its register condition are skipped by uniform tests (RunOp::reg_mask);
-// lanes missing its lane condition are masked off for the product. Each
-// register's update is a branch around it alone, so the registers stay in
-// place from op to op. base = the tile-index bits of the thread's lanes,
-// fixed = the tile's bits outside P.
 // A wave-uniform double through readfirstlane (which takes and returns a
 // 32-bit int: each half is taken as unsigned before it is widened).
 __device__ __forceinline__ double uniform_f64(double x) {
@@ -1074,40 +1083,56 @@ __device__ __forceinline__ double uniform_f64(double x) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// A stretch of `count` consecutive phase ops (tab: their RunPhase entries)
+// on the thread's registers. Whole tiles missing an op's outside bits skip
+// it and registers missing its register condition are skipped (uniform
+// tests); lanes missing its lane condition are masked off. Each register's
+// update is a branch around it alone, so the registers stay in place from op
+// to op (a branch tree over the condition with straight-line leaves made the
+// compiler merge the leaves into one block that indexes the registers at run
+// time, through scratch memory); the four products are taken before either
+// sum. base = the tile-index bits of the thread's lanes, fixed = the tile's
+// bits outside P. Entries load four at a time as 64-bit words: each
+// stretch's entries are padded with no-op entries (register mask 0) to a
+// multiple of four, so a group never reaches past its stretch and needs no
+// count test; two scalar loads and one wait per four ops.
 template <class T, int RPT>
-__device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunOp* opp, int count,
+__device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunPhase* tab, int count,
                                            uint32_t base, uint64_t fixed) {
-  // Groups of G ops: the fields of a group load together (indices clamped to
-  // the stretch), so one wait for the scalar loads covers G ops.
-  constexpr int G = 4;
-  struct Head {  // the first 32 bytes of a RunOp
-    uint64_t out;
-    uint32_t lane, mask;
-    double re, im;
-  };
+  constexpr int G = kRunPhaseGroup;
   for (int q0 = 0; q0 < count; q0 += G) {
     uint32_t lane[G], rm[G];
     ketd::cplx<T> ph[G];
-    Head hd[G];
+    const auto* w = reinterpret_cast<const uint64_t*>(tab + q0);
+    uint64_t hw[4 * G];
 #pragma unroll
-    for (int i = 0; i < G; ++i)
-      hd[i] = *reinterpret_cast<const Head*>(opp + (q0 + i < count ? q0 + i : count - 1));
+    for (int k = 0; k < 4 * G; ++k) hw[k] = w[k];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
       // readfirstlane is convergent, so these values (and their loads) stay
       // here rather than sinking into the op's per-lane branch
-      lane[i] = (uint32_t)__builtin_amdgcn_readfirstlane(hd[i].lane);
-      rm[i] = (uint32_t)__builtin_amdgcn_readfirstlane((fixed & hd[i].out) == hd[i].out ? hd[i].mask
-                                                                                         : 0u);
-      ph[i] = ketd::mk<T>((T)uniform_f64(hd[i].re), (T)uniform_f64(hd[i].im));
+      const uint64_t out = hw[4 * i];
+      lane[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)hw[4 * i + 1]);
+      rm[i] = (uint32_t)__builtin_amdgcn_readfirstlane(
+          (fixed & out) == out ? (int)(uint32_t)(hw[4 * i + 1] >> 32) : 0);
+      ph[i] = ketd::mk<T>((T)uniform_f64(__builtin_bit_cast(double, hw[4 * i + 2])),
+                          (T)uniform_f64(__builtin_bit_cast(double, hw[4 * i + 3])));
     }
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-      if (q0 + i >= count) break;  // uniform
-      if ((base & lane[i]) == lane[i]) {  // per lane
+      if (rm[i] == 0) continue;  // uniform: outside bits, no register, or padding
+      if ((base & lane[i]) == lane[i]) {  // per lane (lane 0: every lane)
 #pragma unroll
         for (int j = 0; j < RPT; ++j)
-          if ((rm[i] >> j) & 1) v[j] = ketd::cmul<T>(v[j], ph[i]);  // uniform
+          if (__builtin_expect((rm[i] >> j) & 1, 1)) {  // uniform
+            const T xy = v[j].x * ph[i].y, yx = v[j].y * ph[i].x;
+            const T xx = v[j].x * ph[i].x, yy = v[j].y * ph[i].y;
+            const T nx = xx - yy;
+            // the real part first: it can overwrite the amplitude's own
+            // registers, and so can the imaginary part after it
+            __builtin_amdgcn_sched_barrier(0);
+            v[j] = ketd::mk<T>(nx, xy + yx);
+          }
       }
     }
   }
@@ -1141,9 +1166,9 @@ __device__ __forceinline__ void reg_op(ketd::cplx<T> (&v)[RPT], const RunOp* opp
 // Moves a run's tile between HBM and LDS in index order (16-byte accesses,
 // consecutive lanes on consecutive addresses); LDS slots by the runs'
 // swizzle (RunRecord::sbit).
-template <class T, int TB>
+template <class T, int TB, int NT>
 struct RunIO {
-  using IO = TileIO<T, TB, kThreads>;
+  using IO = TileIO<T, TB, NT>;
   using V = typename IO::V;
   static constexpr int EPA = IO::EPA, S = IO::S, SB = IO::SB, LS = IO::LS;
   uint64_t g0;
@@ -1197,25 +1222,30 @@ struct RunIO {
 
 // One workgroup per tile (grid-stride): load the tile, apply each segment's
 // ops in registers, store it back.
-template <class T, int TB, bool FMA>
-__global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict__ s,
-                                                        uint64_t n_tiles,
-                                                        const unsigned char* __restrict__ rec) {
-  constexpr int RPT = 1 << (TB - 8);
-  static_assert(RPT <= kRunMaxRegs, "registers per thread");
+template <class T, int TB, int RB, bool FMA>
+__global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile(ketd::cplx<T>* __restrict__ s,
+                                                              uint64_t n_tiles,
+                                                              const unsigned char* __restrict__ rec) {
+  constexpr int RPT = 1 << RB, TBITS = TB - RB, NT = 1 << TBITS;
+  static_assert(RPT <= kRunMaxRegs && TBITS <= kRunMaxThreadBits, "run layout");
   __shared__ ketd::cplx<T> L[1 << TB];
   const RunRecord& h = *reinterpret_cast<const RunRecord*>(rec);
   const auto* ops = reinterpret_cast<const RunOp*>(rec + sizeof(RunRecord));
   const auto* segs = reinterpret_cast<const RunSeg*>(ops + h.n_ops);
-  RunIO<T, TB> io;
+  const auto* phases = reinterpret_cast<const RunPhase*>(rec + h.phase_offset);
+  RunIO<T, TB, NT> io;
   io.init(h, threadIdx.x);
+  // The next tile's loads are issued before this tile's stores, so both are
+  // in flight together.
   for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    typename RunIO<T, TB, NT>::V pv[RunIO<T, TB, NT>::S];
     const uint64_t fixed = deposit_outside(tile, h.pmask);
-    {
-      typename RunIO<T, TB>::V v[RunIO<T, TB>::S];
-      io.gload(s, fixed, v);
-      io.lput(L, v);
-    }
+    // the thread's tile addresses are recomputed each tile rather than held
+    // in registers across the ops (an empty asm that the compiler must
+    // assume changes them)
+    __asm__ volatile("" : "+v"(io.g0), "+v"(io.l0));
+    io.gload(s, fixed, pv);
+    io.lput(L, pv);
     for (int sg = 0; sg < h.n_segs; ++sg) {
       __syncthreads();
       const RunSeg& S = segs[sg];
@@ -1227,7 +1257,7 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
       const auto* rslot = reinterpret_cast<const uint32_t*>(S.rslot);
       uint32_t base = 0, slot = 0;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {  // branch-free, so the tables load as scalars
+      for (int i = 0; i < TBITS; ++i) {  // branch-free, so the tables load as scalars
         const uint32_t on = 0u - ((threadIdx.x >> i) & 1u);
         base |= (tbit[i >> 1] >> (16 * (i & 1))) & 0xffffu & on;
         slot ^= (tslot[i >> 1] >> (16 * (i & 1))) & 0xffffu & on;
@@ -1239,7 +1269,7 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
       for (int o = S.first_op; o < end;) {
         const RunOp* op = ops + o;
         if (op->kind == RUN_PHASE) {
-          reg_phases<T, RPT>(v, op, op->stretch, base, fixed);
+          reg_phases<T, RPT>(v, phases + op->mat_offset, op->stretch, base, fixed);
           o += op->stretch;
         } else {
           reg_op<T, RPT, FMA>(v, op, base, fixed, rec);
@@ -1247,11 +1277,12 @@ __global__ __launch_bounds__(kThreads) void k_run_tile(ketd::cplx<T>* __restrict
         }
       }
       // the slots again (recomputed: not held in registers across the ops)
-      __asm__ volatile("" ::: "memory");
+      __asm__ volatile("" : "+v"(slot)::"memory");
 #pragma unroll
       for (int j = 0; j < RPT; ++j) L[slot ^ ((rslot[j >> 1] >> (16 * (j & 1))) & 0xffffu)] = v[j];
     }
     __syncthreads();
+    __asm__ volatile("" : "+v"(io.g0), "+v"(io.l0));
     io.store(s, fixed, L);
     __syncthreads();
   }
@@ -1336,10 +1367,20 @@ hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void*
   const auto* rec = static_cast<const unsigned char*>(rec_dev);
   auto* d = static_cast<double2*>(s);
   auto* f = static_cast<float2*>(s);
-  if (precision == 0 && contract) k_run_tile<double, 11, true><<<(unsigned)g, kThreads, 0, st>>>(d, n_tiles, rec);
-  else if (precision == 0) k_run_tile<double, 11, false><<<(unsigned)g, kThreads, 0, st>>>(d, n_tiles, rec);
-  else if (contract) k_run_tile<float, 12, true><<<(unsigned)g, kThreads, 0, st>>>(f, n_tiles, rec);
-  else k_run_tile<float, 12, false><<<(unsigned)g, kThreads, 0, st>>>(f, n_tiles, rec);
+  // the layout the planner wrote the records for
+  constexpr int TB0 = 11, RB0 = 3, TB1 = 12, RB1 = 4;
+  if (run_tile_bits(0) != TB0 || run_reg_bits(0) != RB0 || run_tile_bits(1) != TB1 ||
+      run_reg_bits(1) != RB1)
+    return hipErrorInvalidValue;
+  constexpr unsigned NT0 = 1u << (TB0 - RB0), NT1 = 1u << (TB1 - RB1);
+  if (precision == 0 && contract)
+    k_run_tile<double, TB0, RB0, true><<<(unsigned)g, NT0, 0, st>>>(d, n_tiles, rec);
+  else if (precision == 0)
+    k_run_tile<double, TB0, RB0, false><<<(unsigned)g, NT0, 0, st>>>(d, n_tiles, rec);
+  else if (contract)
+    k_run_tile<float, TB1, RB1, true><<<(unsigned)g, NT1, 0, st>>>(f, n_tiles, rec);
+  else
+    k_run_tile<float, TB1, RB1, false><<<(unsigned)g, NT1, 0, st>>>(f, n_tiles, rec);
   return hipGetLastError();
 }
 

@@ -135,14 +135,19 @@ template <class T>
 void reg_op(std::vector<C<T>>& v, const ketk::RunOp& op, uint32_t base, uint64_t fixed,
             const unsigned char* rec) {
   const int rpt = (int)v.size();
-  if ((fixed & op.out_mask) != op.out_mask) return;
-  if ((base & op.reg_lane) != op.reg_lane) return;
-  if (op.kind == ketk::RUN_PHASE) {
-    const C<T> ph{(T)op.m[0], (T)op.m[1]};
+  if (op.kind == ketk::RUN_PHASE) {  // read from the op's RunPhase, as the kernel does
+    ketk::RunRecord h;
+    std::memcpy(&h, rec, sizeof h);
+    ketk::RunPhase p;
+    std::memcpy(&p, rec + h.phase_offset + op.mat_offset * sizeof p, sizeof p);
+    if ((fixed & p.out_mask) != p.out_mask || (base & p.reg_lane) != p.reg_lane) return;
+    const C<T> ph{(T)p.re, (T)p.im};
     for (int j = 0; j < rpt; ++j)
-      if ((op.reg_mask >> j) & 1) v[j] = cmul(v[j], ph);
+      if ((p.reg_mask >> j) & 1) v[j] = cmul(v[j], ph);
     return;
   }
+  if ((fixed & op.out_mask) != op.out_mask) return;
+  if ((base & op.reg_lane) != op.reg_lane) return;
   if (op.kind == ketk::RUN_MATRIX) {
     const int D = 1 << op.npos;
     const unsigned char* mat = rec + op.mat_offset;
@@ -172,6 +177,10 @@ void reg_op(std::vector<C<T>>& v, const ketk::RunOp& op, uint32_t base, uint64_t
     if (op.kind == ketk::RUN_DENSE) {
       x = cadd(cmul(m00, a), cmul(m01, b));
       y = cadd(cmul(m10, a), cmul(m11, b));
+    } else if (op.kind == ketk::RUN_DENSE_REAL) {  // real entries: real products
+      if (m00.y != 0 || m01.y != 0 || m10.y != 0 || m11.y != 0) bad_record("real dense op");
+      x = {m00.x * a.x + m01.x * b.x, m00.x * a.y + m01.x * b.y};
+      y = {m10.x * a.x + m11.x * b.x, m10.x * a.y + m11.x * b.y};
     } else if (op.kind == ketk::RUN_CH) {
       const T r = m00.x;
       x = {(a.x + b.x) * r, (a.y + b.y) * r};
@@ -191,9 +200,34 @@ void reg_op(std::vector<C<T>>& v, const ketk::RunOp& op, uint32_t base, uint64_t
 // register form of every op tests exactly the bits of its local form.
 // Returns the LDS bank-row conflicts of the segments' accesses.
 uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& ops,
-                      const std::vector<ketk::RunSeg>& segs, int tb, int row_bits) {
-  const int rpt = 1 << (tb - 8);
+                      const std::vector<ketk::RunSeg>& segs, int tb, int rb, int row_bits,
+                      const unsigned char* rec) {
+  const int rpt = 1 << rb, n_tb = tb - rb;
   const uint32_t all = (1u << tb) - 1;
+  // the phase table: after the segments, each phase stretch's entries with
+  // its ops' fields in op order, then zero entries up to a whole group
+  if (h.phase_offset != sizeof h + ops.size() * sizeof(ketk::RunOp) +
+                            segs.size() * sizeof(ketk::RunSeg))
+    bad_record("phase table offset");
+  uint32_t n_phase = 0;
+  for (std::size_t o = 0; o < ops.size(); ++o) {
+    const ketk::RunOp& op = ops[o];
+    if (op.kind != ketk::RUN_PHASE) continue;
+    if (op.mat_offset != n_phase) bad_record("phase table index");
+    ketk::RunPhase p;
+    std::memcpy(&p, rec + h.phase_offset + n_phase * sizeof p, sizeof p);
+    if (p.out_mask != op.out_mask || p.reg_lane != op.reg_lane || p.reg_mask != op.reg_mask ||
+        p.re != op.m[0] || p.im != op.m[1])
+      bad_record("phase table entry");
+    ++n_phase;
+    if (op.stretch == 1)  // the stretch ends: padding to a whole group
+      for (; n_phase % ketk::kRunPhaseGroup; ++n_phase) {
+        unsigned char z[sizeof(ketk::RunPhase)];
+        std::memcpy(z, rec + h.phase_offset + n_phase * sizeof(ketk::RunPhase), sizeof z);
+        for (unsigned char c : z)
+          if (c) bad_record("phase table padding");
+      }
+  }
   for (int b = 0; b < tb; ++b)
     if (h.sbit[b] != ketk::run_slot(1u << b, row_bits)) bad_record("slot table");
   int next = 0;
@@ -202,16 +236,16 @@ uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& 
     if (sg.first_op != next || sg.n_ops < 1) bad_record("segments do not tile the ops");
     next += sg.n_ops;
     uint32_t tbits = 0;
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < n_tb; ++i) {
       if (__builtin_popcount(sg.tbit[i]) != 1 || (tbits & sg.tbit[i])) bad_record("thread bits");
       tbits |= sg.tbit[i];
       if (sg.tslot[i] != ketk::run_slot(sg.tbit[i], row_bits)) bad_record("thread slots");
     }
     if ((tbits & sg.rmask) || (tbits | sg.rmask) != all ||
-        __builtin_popcount(sg.rmask) != tb - 8)
+        __builtin_popcount(sg.rmask) != rb)
       bad_record("layout is not a partition of the tile bits");
-    int rpos[4], nr = 0;
-    for (int r = 0; r < tb - 8; ++r) {
+    int rpos[5], nr = 0;
+    for (int r = 0; r < rb; ++r) {
       const uint32_t b = sg.roff[1u << r];
       if (__builtin_popcount(b) != 1 || !(b & sg.rmask)) bad_record("register bits");
       rpos[nr++] = __builtin_ctz(b);
@@ -223,9 +257,9 @@ uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& 
       if (sg.roff[j] != t || sg.rslot[j] != ketk::run_slot(t, row_bits)) bad_record("register slots");
     }
     std::vector<int> seen(std::size_t{1} << tb, 0);
-    for (uint32_t tid = 0; tid < 256; ++tid) {
+    for (uint32_t tid = 0; tid < (1u << n_tb); ++tid) {
       uint32_t base = 0, slot = 0;
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < n_tb; ++i)
         if ((tid >> i) & 1) {
           base |= sg.tbit[i];
           slot ^= sg.tslot[i];
@@ -242,7 +276,7 @@ uint64_t check_record(const ketk::RunRecord& h, const std::vector<ketk::RunOp>& 
         std::vector<int> rows(1u << row_bits, 0);
         for (uint32_t l = g; l < g + (1u << row_bits); ++l) {
           uint32_t slot = 0;
-          for (int i = 0; i < 8; ++i)
+          for (int i = 0; i < n_tb; ++i)
             if ((l >> i) & 1) slot ^= sg.tslot[i];
           if (rows[(slot ^ sg.rslot[j]) & ((1u << row_bits) - 1)]++) ++conflicts;
         }
@@ -295,10 +329,10 @@ void run_record(C<T>* s, int n, int precision, const unsigned char* rec) {
   std::vector<ketk::RunSeg> segs(h.n_segs);
   std::memcpy(segs.data(), rec + sizeof h + ops.size() * sizeof(ketk::RunOp),
               segs.size() * sizeof(ketk::RunSeg));
-  const int tb = ketk::run_tile_bits(precision);
+  const int tb = ketk::run_tile_bits(precision), rb = ketk::run_reg_bits(precision);
   const int row_bits = precision == 0 ? 3 : 4;
-  const int rpt = 1 << (tb - 8);
-  g_bank_conflicts += check_record(h, ops, segs, tb, row_bits);
+  const int rpt = 1 << rb, tbits = tb - rb;
+  g_bank_conflicts += check_record(h, ops, segs, tb, rb, row_bits, rec);
   std::vector<C<T>> L(std::size_t{1} << tb);
   std::vector<C<T>> v(rpt);
   for (uint64_t tile = 0; tile < (uint64_t{1} << (n - tb)); ++tile) {
@@ -306,9 +340,9 @@ void run_record(C<T>* s, int n, int precision, const unsigned char* rec) {
     for (uint64_t t = 0; t < L.size(); ++t)
       L[ketk::run_slot((uint32_t)t, row_bits)] = s[fixed | deposit(t, h.pmask)];
     for (const ketk::RunSeg& sg : segs)
-      for (uint32_t tid = 0; tid < 256; ++tid) {
+      for (uint32_t tid = 0; tid < (1u << tbits); ++tid) {
         uint32_t base = 0, slot = 0;
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < tbits; ++i)
           if ((tid >> i) & 1) {
             base |= sg.tbit[i];
             slot ^= sg.tslot[i];
@@ -480,7 +514,7 @@ int learned(const std::vector<ket_gate>& g, int n, int precision) {
       std::memcpy(segs.data(), rec + sizeof h + ops.size() * sizeof(ketk::RunOp),
                   segs.size() * sizeof(ketk::RunSeg));
       g_bank_conflicts += check_record(h, ops, segs, ketk::run_tile_bits(precision),
-                                       precision == 0 ? 3 : 4);
+                                       ketk::run_reg_bits(precision), precision == 0 ? 3 : 4, rec);
     }
     blocks += o.kind == ketplan::Kind::Block;
   }
