@@ -339,6 +339,11 @@ typedef struct ket_tuning {
                                keeps, so its HBM rows are 2^run_low_bits
                                amplitudes long (3..10; default 3 complex128,
                                4 complex64) */
+  int32_t device_build;     /* matrices of fused blocks and of ket's Fuser
+                               blocks: 1 (default) built on the GPU in FP64
+                               after the plan's upload, one thread per
+                               column; 0 built on the host. Both use the same
+                               products and sums and are equal bit for bit */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

@@ -182,6 +182,11 @@ int run_plan(ket_sv* sv, const ketplan::Plan& plan, bool pool_current = false) {
     HIP_TRY(hipEventRecord(sv->pool_ev, sv->stream), "hipEventRecord(pool)");
     pool_dev = static_cast<const unsigned char*>(sv->pool);
     ++sv->pool_uploads;
+    // block matrices the GPU builds into the pool just uploaded (a cached
+    // pool already holds them)
+    HIP_TRY(ketk::launch_build(sv->stream, sv->pool, plan.build_table, (int)plan.builds.size(),
+                               plan.build_kmax),
+            "k_build");
   }
   for (const ketplan::Op& o : plan.ops) {
     int st = launch_op(sv, o, pool_dev);
@@ -800,9 +805,12 @@ int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
     return fail(KET_ERR_ARG, "ket_amd: exact_runs must be 0 or 1");
   if (t->run_low_bits != 0 && (t->run_low_bits < 3 || t->run_low_bits > 10))
     return fail(KET_ERR_ARG, "ket_amd: run_low_bits must be 0 or in [3, 10]");
+  if (t->device_build != 0 && t->device_build != 1)
+    return fail(KET_ERR_ARG, "ket_amd: device_build must be 0 or 1");
   *out = {t->items_per_thread, t->nontemporal, t->blocks_per_cu,
           t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
-          t->block_stage, t->pair_exchange, t->block_k, t->exact_runs, t->run_low_bits};
+          t->block_stage, t->pair_exchange, t->block_k, t->exact_runs, t->run_low_bits,
+          t->device_build};
   return KET_OK;
 }
 
@@ -817,6 +825,7 @@ void fill_tuning(const ketk::Tuning& k, ket_tuning* t) {
   t->block_k = k.block_k;
   t->exact_runs = k.exact_runs;
   t->run_low_bits = k.run_low_bits;
+  t->device_build = k.device_build;
 }
 }  // namespace
 

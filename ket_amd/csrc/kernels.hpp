@@ -40,6 +40,8 @@ struct Tuning {
   int exact_runs;    // group consecutive exact passes into runs (ket's Fuser,
                      // and the passes the block packer leaves)
   int run_low_bits;  // exact runs: low index bits kept in every tile
+  int device_build;  // fused and exact block matrices built on the GPU (1)
+                     // or on the host (0) from the same products
 };
 // Per precision (0 = complex128, 1 = complex64).
 void set_tuning(int precision, const Tuning& t);
@@ -169,6 +171,49 @@ inline int run_thread_bits(int precision) { return run_tile_bits(precision) - ru
 // equals the passes' within rounding only); false: bit for bit.
 hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
                       bool contract = false);
+
+// Block matrices built on the GPU (sv_build.hip). A build record is a
+// BuildHeader, n_ops BuildOps, n_folds BuildFolds; the plan's build table
+// (Plan::build_table, n Plan::builds) lists the records' pool offsets. One
+// workgroup per record, one thread per matrix column: the column starts as
+// the basis vector e_col and every op updates it in order with the host
+// builder's products and sums (kethost::block_matrix), then every fold
+// multiplies the rows holding its mask by its phase (op_for_block); the
+// result is written row-major at `dest`, as complex128 or converted to
+// complex64, equal bit for bit to the host-built matrix.
+enum BuildKind {
+  BUILD_SINGLE = 0,  // rows r with (r & cm) == cm and !(r & tm): (x, y) of
+                     // rows (r, r | tm) -> (m0 x + m1 y, m2 x + m3 y)
+  BUILD_CH = 1,      // as SINGLE with ((x + y) r, (x - y) r), r = m[0]
+  BUILD_CY = 2,      // as SINGLE with ((0, -1) y, (0, 1) x)
+  BUILD_FLIP = 3,    // swap rows r, r | tm for (r & cm) == cm, !(r & tm)
+  BUILD_SWAP = 4,    // swap rows r, (r ^ tm) | xm for (r & cm) == cm,
+                     // (r & tm), !(r & xm)
+  BUILD_PHASE = 5    // rows r with (r & tm) == tm: row * (m[0], m[1])
+};
+struct BuildOp {
+  int32_t kind;
+  uint32_t tm, cm, xm;  // local row masks (see BuildKind)
+  double m[8];          // 2x2 row-major (re, im); phase: m[0], m[1]
+};
+struct BuildFold {      // rows r with (r & lm) == lm: (c, s) * row
+  uint32_t lm;
+  int32_t pad;
+  double c, s;
+};
+struct BuildHeader {
+  uint32_t dest;     // pool offset of the matrix
+  int32_t k;         // width: a 2^k x 2^k matrix
+  int32_t n_ops;
+  int32_t n_folds;
+  int32_t f32;       // 1: written as complex64
+  int32_t pad[3];
+};
+static_assert(sizeof(BuildOp) == 80 && sizeof(BuildFold) == 24 && sizeof(BuildHeader) == 32,
+              "build record layout");
+// Builds the `count` matrices whose records the table at pool + table lists
+// (widths up to kmax <= 6).
+hipError_t launch_build(hipStream_t st, void* pool_dev, uint32_t table, int count, int kmax);
 
 // Pool record of a fused block (see sv_block.hip). Matrix bit b of an
 // element index is qubit qubits[b]; offsets[e] = that element's index bits.

@@ -462,6 +462,10 @@ int run_low_bits(int precision) {
 
 }  // namespace
 
+namespace {
+void copy_build(Plan& plan, std::size_t from, std::size_t dest, bool move);
+}  // namespace
+
 void group_runs(Plan& plan, int n, int precision, bool contract) {
   const int tb = ketk::run_tile_bits(precision);
   if (n < tb) return;
@@ -520,8 +524,15 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
     const std::size_t phase_at = sizeof(ketk::RunRecord) + ops.size() * sizeof(ketk::RunOp) +
                                  segs.size() * sizeof(ketk::RunSeg);
     const std::size_t head = phase_at + phases.size() * sizeof(ketk::RunPhase);
-    for (auto& r : ops)
-      if (r.kind == ketk::RUN_MATRIX) r.mat_offset += (uint32_t)head;
+    // (matrix op's pass, its matrix's offset in the record) for matrices the
+    // GPU builds: the run's copy of the matrix is built there instead
+    std::vector<std::pair<std::size_t, std::size_t>> built;
+    for (std::size_t t = 0; t < ops.size(); ++t) {
+      auto& r = ops[t];
+      if (r.kind != ketk::RUN_MATRIX) continue;
+      r.mat_offset += (uint32_t)head;
+      if (in[i + t].build_at >= 0) built.push_back({i + t, r.mat_offset});
+    }
     const std::size_t off = (plan.pool.size() + 255) & ~std::size_t{255};
     plan.pool.resize(off + head + mats.size());
     ketk::RunRecord h{};
@@ -538,6 +549,8 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
     if (!phases.empty())
       std::memcpy(dst + phase_at, phases.data(), phases.size() * sizeof(ketk::RunPhase));
     if (!mats.empty()) std::memcpy(dst + head, mats.data(), mats.size());
+    for (const auto& b : built)
+      copy_build(plan, (std::size_t)in[b.first].build_at, off + b.second, true);
     Op o;
     o.kind = Kind::Run;
     o.pool_offset = off;
@@ -587,11 +600,20 @@ void fuser_flush(std::vector<const ket_gate*>& pend, std::vector<int>& bq,
           if (bq[b] == g->qubits[i]) r.local[i] = b;
       refs.push_back(r);
     }
-    const std::vector<cx> mat = kethost::block_matrix(refs, m);
     if (m == 1) {
+      const std::vector<cx> mat = kethost::block_matrix(refs, m);
       plan->ops.push_back(pair_op(n, line_bits(precision), ketk::OP_DENSE,
                                   bq[0], {}, mat.data()));
+    } else if (plan->device_build) {
+      // the record with its matrix left for the GPU to build
+      const std::size_t D = std::size_t{1} << m;
+      const std::vector<double> zero(2 * D * D, 0.0);
+      plan->ops.push_back(op_for_matrix(*plan, n, precision, m, bq.data(), zero.data(),
+                                        /*exact=*/true));
+      plan->ops.back().build_at =
+          (std::int64_t)emit_build(*plan, m, refs, {}, plan->ops.back().pool_offset, false);
     } else {
+      const std::vector<cx> mat = kethost::block_matrix(refs, m);
       plan->ops.push_back(op_for_matrix(*plan, n, precision, m, bq.data(),
                                         &mat[0].re, /*exact=*/true));
     }
@@ -628,8 +650,129 @@ bool diag_terms(const ket_gate& g, std::vector<Term>* out) {
   }
 }
 
+std::size_t emit_build(Plan& plan, int k, const std::vector<kethost::GateRef>& gates,
+                       const std::vector<ketk::BuildFold>& folds, std::size_t dest, bool f32) {
+  // The ops as kethost::block_matrix applies them (same masks and matrices).
+  std::vector<ketk::BuildOp> ops;
+  for (const kethost::GateRef& g : gates) {
+    const int* on = g.local;
+    ketk::BuildOp o{};
+    cx mm[4];
+    auto set_m = [&](const cx* m) {
+      for (int e = 0; e < 4; ++e) {
+        o.m[2 * e] = m[e].re;
+        o.m[2 * e + 1] = m[e].im;
+      }
+    };
+    switch (g.type) {
+      case KET_H: case KET_X: case KET_Y: case KET_Z: case KET_S: case KET_SDG:
+      case KET_T: case KET_TDG: case KET_RX: case KET_RY: case KET_RZ: case KET_U:
+        o.kind = ketk::BUILD_SINGLE;
+        o.tm = 1u << on[0];
+        if (g.mat2) {
+          set_m(g.mat2);
+        } else {
+          kethost::single_matrix(g.type, g.params, mm);
+          set_m(mm);
+        }
+        break;
+      case KET_CRX: case KET_CRY: case KET_CRZ: case KET_CU:
+        o.kind = ketk::BUILD_SINGLE;
+        o.tm = 1u << on[1];
+        o.cm = 1u << on[0];
+        kethost::single_matrix(g.type, g.params, mm);
+        set_m(mm);
+        break;
+      case KET_CH:
+        o.kind = ketk::BUILD_CH;
+        o.cm = 1u << on[0];
+        o.tm = 1u << on[1];
+        o.m[0] = 1.0 / std::sqrt(2.0);
+        break;
+      case KET_CX: case KET_CCX: {
+        const int nc = g.type == KET_CX ? 1 : 2;
+        o.kind = ketk::BUILD_FLIP;
+        for (int i = 0; i < nc; ++i) o.cm |= 1u << on[i];
+        o.tm = 1u << on[nc];
+        break;
+      }
+      case KET_CY:
+        o.kind = ketk::BUILD_CY;
+        o.cm = 1u << on[0];
+        o.tm = 1u << on[1];
+        break;
+      case KET_CZ: case KET_CP: {
+        o.kind = ketk::BUILD_PHASE;
+        o.tm = (1u << on[0]) | (1u << on[1]);
+        if (g.type == KET_CZ) {
+          o.m[0] = -1.0;
+        } else {
+          const kethost::CosSin t = kethost::cos_sin(g.params[0]);
+          o.m[0] = t.c;
+          o.m[1] = t.s;
+        }
+        break;
+      }
+      case KET_SWAP: case KET_CSWAP: {
+        const int b = g.type == KET_SWAP ? 0 : 1;
+        o.kind = ketk::BUILD_SWAP;
+        o.cm = g.type == KET_SWAP ? 0u : 1u << on[0];
+        o.tm = 1u << on[b];
+        o.xm = 1u << on[b + 1];
+        break;
+      }
+      default:
+        continue;
+    }
+    ops.push_back(o);
+  }
+  ketk::BuildHeader h{};
+  h.dest = (uint32_t)dest;
+  h.k = k;
+  h.n_ops = (int32_t)ops.size();
+  h.n_folds = (int32_t)folds.size();
+  h.f32 = f32 ? 1 : 0;
+  const std::size_t bytes = sizeof h + ops.size() * sizeof(ketk::BuildOp) +
+                            folds.size() * sizeof(ketk::BuildFold);
+  const std::size_t at = (plan.pool.size() + 15) & ~std::size_t{15};
+  plan.pool.resize(at + bytes);
+  unsigned char* q = plan.pool.data() + at;
+  std::memcpy(q, &h, sizeof h);
+  if (!ops.empty()) std::memcpy(q + sizeof h, ops.data(), ops.size() * sizeof(ketk::BuildOp));
+  if (!folds.empty())
+    std::memcpy(q + sizeof h + ops.size() * sizeof(ketk::BuildOp), folds.data(),
+                folds.size() * sizeof(ketk::BuildFold));
+  plan.builds.push_back((uint32_t)at);
+  plan.build_kmax = std::max(plan.build_kmax, k);
+  return at;
+}
+
+namespace {
+
+// A copy of the build record at `from` that writes its matrix at `dest`
+// instead; the record at `from` is dropped from the builds when `move`.
+void copy_build(Plan& plan, std::size_t from, std::size_t dest, bool move) {
+  ketk::BuildHeader h;
+  std::memcpy(&h, plan.pool.data() + from, sizeof h);
+  const std::size_t bytes = sizeof h + (std::size_t)h.n_ops * sizeof(ketk::BuildOp) +
+                            (std::size_t)h.n_folds * sizeof(ketk::BuildFold);
+  const std::size_t at = (plan.pool.size() + 15) & ~std::size_t{15};
+  plan.pool.resize(at + bytes);
+  std::memcpy(plan.pool.data() + at, plan.pool.data() + from, bytes);
+  h.dest = (uint32_t)dest;
+  std::memcpy(plan.pool.data() + at, &h, sizeof h);
+  if (move) {
+    auto it = std::find(plan.builds.begin(), plan.builds.end(), (uint32_t)from);
+    if (it != plan.builds.end()) plan.builds.erase(it);
+  }
+  plan.builds.push_back((uint32_t)at);
+}
+
+}  // namespace
+
 Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
-                const double* mat_in, const std::vector<Term>& terms_in) {
+                const double* mat_in, const std::vector<Term>& terms_in,
+                const std::vector<kethost::GateRef>* dev_gates) {
   (void)n;
   const std::size_t D = std::size_t{1} << k;
   uint64_t Q = 0;
@@ -639,10 +782,14 @@ Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
       if (qubits[b] == qb) return b;
     return -1;
   };
-  // Fold terms with every bit inside the block into the matrix rows; sort
-  // the rest by the block bit they depend on.
-  std::vector<cx> mat(D * D);
-  std::memcpy(mat.data(), mat_in, D * D * sizeof(cx));
+  // Fold terms with every bit inside the block into the matrix rows (here,
+  // or by the GPU build); sort the rest by the block bit they depend on.
+  std::vector<cx> mat;
+  std::vector<ketk::BuildFold> folds;
+  if (!dev_gates) {
+    mat.resize(D * D);
+    std::memcpy(mat.data(), mat_in, D * D * sizeof(cx));
+  }
   std::vector<std::vector<Term>> byj(k + 1);
   for (const Term& t : terms_in) {
     const uint64_t in = t.mask & Q, out = t.mask & ~Q;
@@ -652,6 +799,10 @@ Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
       for (int qb = 0; qb < 64; ++qb)
         if (in & bit(qb)) lm |= bit(local(qb));
       const kethost::CosSin ph = kethost::cos_sin(t.angle);
+      if (dev_gates) {
+        folds.push_back({(uint32_t)lm, 0, ph.c, ph.s});
+        continue;
+      }
       for (std::size_t r = 0; r < D; ++r)
         if ((r & lm) == lm)
           for (std::size_t c = 0; c < D; ++c)
@@ -700,7 +851,10 @@ Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
       ta[i] = t.angle;
       ++i;
     }
-  if (f32) {
+  if (dev_gates) {
+    std::memset(p + h.mat_offset, 0, mat_bytes);
+    emit_build(plan, k, *dev_gates, folds, off + h.mat_offset, f32);  // may move p
+  } else if (f32) {
     float* fm = reinterpret_cast<float*>(p + h.mat_offset);
     for (std::size_t e = 0; e < D * D; ++e) {
       fm[2 * e] = (float)mat[e].re;
@@ -903,16 +1057,23 @@ void pack_blocks(const ket_gate* gates, const std::vector<std::size_t>& run,
       refs.push_back(r);
     }
     std::vector<std::array<cx, 4>> merged;
-    const std::vector<cx> mat = kethost::block_matrix(kethost::merge_single_runs(refs, merged), k);
-    const std::size_t pool_mark = plan->pool.size();
-    Op block = (k == 1 && trailing.empty() && !force_blocks)
-                   ? pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data())
-                   : op_for_block(*plan, n, precision, k, bq.data(), &mat[0].re, terms);
+    const std::vector<kethost::GateRef> mrefs = kethost::merge_single_runs(refs, merged);
+    const bool pair = k == 1 && trailing.empty() && !force_blocks;
+    const bool dev = plan->device_build && !pair;  // the GPU builds the matrix
+    std::vector<cx> mat;
+    if (!dev) mat = kethost::block_matrix(mrefs, k);
+    const std::size_t pool_mark = plan->pool.size(), build_mark = plan->builds.size();
+    const int kmax_mark = plan->build_kmax;
+    Op block = pair ? pair_op(n, L, ketk::OP_DENSE, bq[0], {}, mat.data())
+                    : op_for_block(*plan, n, precision, k, bq.data(),
+                                   dev ? nullptr : &mat[0].re, terms, dev ? &mrefs : nullptr);
     block.n_gates = (int)order.size();
     const bool use_block =
         force_blocks || (decide ? decide(ch, block, singles) : default_decider(ch));
     if (!use_block) {
       plan->pool.resize(pool_mark);  // drop the unused record
+      plan->builds.resize(build_mark);
+      plan->build_kmax = kmax_mark;
       for (const Op& o : singles) plan->ops.push_back(o);
       continue;
     }
@@ -1140,11 +1301,37 @@ bool learned_plan(const ket_gate* gates, std::size_t n_gates, int n, int precisi
 
 }  // namespace
 
+namespace {
+
+int plan_circuit_ops(const ket_gate* gates, std::size_t n_gates, int n, int precision,
+                     int fusion, int block_k, Plan* out, std::string* err);
+
+// The build table at the end of the pool (after every record).
+void finish_builds(Plan& plan) {
+  if (plan.builds.empty()) return;
+  const std::size_t at = (plan.pool.size() + 15) & ~std::size_t{15};
+  plan.pool.resize(at + plan.builds.size() * sizeof(uint32_t));
+  std::memcpy(plan.pool.data() + at, plan.builds.data(), plan.builds.size() * sizeof(uint32_t));
+  plan.build_table = (uint32_t)at;
+}
+
+}  // namespace
+
 int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
                  int precision, int fusion, int block_k, Plan* out,
                  std::string* err) {
+  const int st = plan_circuit_ops(gates, n_gates, n, precision, fusion, block_k, out, err);
+  if (st == KET_OK) finish_builds(*out);
+  return st;
+}
+
+namespace {
+
+int plan_circuit_ops(const ket_gate* gates, std::size_t n_gates, int n, int precision,
+                     int fusion, int block_k, Plan* out, std::string* err) {
   Plan& plan = *out;
   plan = Plan{};
+  plan.device_build = ketk::get_tuning(precision).device_build != 0;
   for (std::size_t i = 0; i < n_gates; ++i) {
     const int st = validate(gates[i], n, err);
     if (st != KET_OK) return st;
@@ -1193,5 +1380,7 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
   *err = "ket_amd: unknown fusion mode " + std::to_string(fusion);
   return KET_ERR_ARG;
 }
+
+}  // namespace
 
 }  // namespace ketplan

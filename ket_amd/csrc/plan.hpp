@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "hostmath.hpp"
 #include "kernels.hpp"
 #include "ket_amd.h"
 
@@ -33,6 +34,8 @@ struct Op {
   int run_matrix_ops = 0;                   // Run: exact-matrix passes among them
   int run_perm_ops = 0;                     // Run: permutation passes (X, SWAP, ...)
   bool run_contract = false;                // Run: fused multiply-adds (see group_runs)
+  std::int64_t build_at = -1;               // MatrixExact: pool offset of the build
+                                            // record of its matrix (GPU-built), or -1
 };
 
 // A diagonal phase term: multiply by exp(i * angle) where all bits of mask
@@ -51,6 +54,14 @@ struct Plan {
   std::vector<unsigned char> pool;  // matrices and index tables, uploaded once
   std::uint64_t gates = 0;
   std::uint64_t blocks = 0;
+  // Matrices the GPU builds after the upload (ketk::BuildHeader records):
+  // the records' pool offsets while planning; the table of them in the pool
+  // (build_table) once planning is done, widths up to build_kmax.
+  std::vector<std::uint32_t> builds;
+  std::uint32_t build_table = 0;
+  int build_kmax = 0;
+  bool device_build = false;  // block matrices as build records (plan_circuit
+                              // sets it from ketk::Tuning::device_build)
 };
 
 // Width of a 128-byte line in index bits: 3 for complex128, 4 for complex64.
@@ -98,10 +109,20 @@ int plan_circuit(const ket_gate* gates, std::size_t n_gates, int n,
                  int precision, int fusion, int block_k, Plan* out,
                  std::string* err);
 
-// A fused block pass: matrix bit b = qubits[b], trailing phase terms.
+// A fused block pass: matrix bit b = qubits[b], trailing phase terms. With
+// dev_gates (block-local qubits), mat is not read: the record's matrix is
+// left for the GPU to build from those gates (a build record, Plan::builds).
 Op op_for_block(Plan& plan, int n, int precision, int k, const int* qubits,
                 const double* mat /* 4^k complex128 */,
-                const std::vector<Term>& terms);
+                const std::vector<Term>& terms,
+                const std::vector<kethost::GateRef>* dev_gates = nullptr);
+
+// Appends a build record for the 2^k x 2^k matrix of `gates` (block-local
+// qubits, kethost::block_matrix's arithmetic) and then `folds`, written at
+// pool offset `dest` as complex128 or complex64 (f32).
+// Returns the record's pool offset.
+std::size_t emit_build(Plan& plan, int k, const std::vector<kethost::GateRef>& gates,
+                       const std::vector<ketk::BuildFold>& folds, std::size_t dest, bool f32);
 
 // Decides whether a packed block is emitted as one block pass (true) or as
 // its gates' own passes; the default compares algorithmic bytes, the learned

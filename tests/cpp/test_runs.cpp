@@ -8,7 +8,7 @@
 // (check_record).
 //
 //   test_runs <records.bin> <n> <precision 0|1>
-//             <mode: none | reference | runs | features | learned> [run_low_bits]
+//             <mode: none | reference | runs | features | learned | build> [run_low_bits]
 //
 // mode none compares KET_FUSE_NONE with KET_FUSE_EXACT; mode reference
 // compares KET_FUSE_REFERENCE with exact runs off and on. Prints
@@ -21,7 +21,10 @@
 // matrices) with those of the plans it then executes. Mode learned plans
 // KET_FUSE_LEARNED twice (with whatever model the build provides) and
 // checks that both plans are equal and every run record well formed;
-// prints "ok learned <passes> <runs> <blocks>".
+// prints "ok learned <passes> <runs> <blocks>". Mode build compares block
+// matrices built on the host with the build records the GPU would run
+// (emulated here with k_build's arithmetic), byte for byte. Plans whose
+// matrices the GPU builds are executed after that emulation.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -356,8 +359,87 @@ void run_record(C<T>* s, int n, int precision, const unsigned char* rec) {
   }
 }
 
+// k_build (sv_build.hip) on the CPU: every build record of the plan, one
+// column at a time, with the kernel's products and sums, written into the
+// pool as the GPU writes it into the uploaded copy.
+void run_builds(ketplan::Plan& plan) {
+  using D2 = C<double>;
+  auto mulc = [](D2 a, D2 b) { return D2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; };
+  if (plan.builds.empty()) return;
+  std::vector<uint32_t> table(plan.builds.size());
+  std::memcpy(table.data(), plan.pool.data() + plan.build_table, table.size() * sizeof(uint32_t));
+  if (table != plan.builds) bad_record("build table");
+  for (uint32_t at : table) {
+    ketk::BuildHeader h;
+    std::memcpy(&h, plan.pool.data() + at, sizeof h);
+    if (h.k < 1 || h.k > plan.build_kmax) bad_record("build width");
+    std::vector<ketk::BuildOp> ops(h.n_ops);
+    std::memcpy(ops.data(), plan.pool.data() + at + sizeof h, ops.size() * sizeof(ketk::BuildOp));
+    std::vector<ketk::BuildFold> folds(h.n_folds);
+    std::memcpy(folds.data(), plan.pool.data() + at + sizeof h + ops.size() * sizeof(ketk::BuildOp),
+                folds.size() * sizeof(ketk::BuildFold));
+    const uint32_t d = 1u << h.k;
+    if (h.dest + (std::size_t)d * d * (h.f32 ? 8 : 16) > plan.pool.size()) bad_record("build dest");
+    for (uint32_t c = 0; c < d; ++c) {
+      std::vector<D2> S(d, D2{0.0, 0.0});
+      S[c] = {1.0, 0.0};
+      for (const ketk::BuildOp& o : ops) {
+        const uint32_t tm = o.tm, cm = o.cm, xm = o.xm;
+        const D2 m0{o.m[0], o.m[1]}, m1{o.m[2], o.m[3]}, m2{o.m[4], o.m[5]}, m3{o.m[6], o.m[7]};
+        for (uint32_t r = 0; r < d; ++r) {
+          switch (o.kind) {
+            case ketk::BUILD_SINGLE:
+            case ketk::BUILD_CH:
+            case ketk::BUILD_CY:
+            case ketk::BUILD_FLIP: {
+              if ((r & tm) || (r & cm) != cm) break;
+              const D2 x = S[r], y = S[r | tm];
+              if (o.kind == ketk::BUILD_SINGLE) {
+                S[r] = cadd(mulc(m0, x), mulc(m1, y));
+                S[r | tm] = cadd(mulc(m2, x), mulc(m3, y));
+              } else if (o.kind == ketk::BUILD_CH) {
+                S[r] = {(x.x + y.x) * o.m[0], (x.y + y.y) * o.m[0]};
+                S[r | tm] = {(x.x - y.x) * o.m[0], (x.y - y.y) * o.m[0]};
+              } else if (o.kind == ketk::BUILD_CY) {
+                S[r] = mulc(D2{0.0, -1.0}, y);
+                S[r | tm] = mulc(D2{0.0, 1.0}, x);
+              } else {
+                S[r] = y;
+                S[r | tm] = x;
+              }
+              break;
+            }
+            case ketk::BUILD_SWAP:
+              if ((r & cm) != cm || !(r & tm) || (r & xm)) break;
+              std::swap(S[r], S[(r ^ tm) | xm]);
+              break;
+            case ketk::BUILD_PHASE:
+              if ((r & tm) == tm) S[r] = mulc(S[r], m0);
+              break;
+            default:
+              bad_record("build op kind");
+          }
+        }
+      }
+      for (const ketk::BuildFold& f : folds)
+        for (uint32_t r = 0; r < d; ++r)
+          if ((r & f.lm) == f.lm) S[r] = mulc(D2{f.c, f.s}, S[r]);
+      for (uint32_t r = 0; r < d; ++r) {
+        unsigned char* q = plan.pool.data() + h.dest;
+        if (h.f32) {
+          const float v[2] = {(float)S[r].x, (float)S[r].y};
+          std::memcpy(q + (std::size_t)(r * d + c) * 8, v, 8);
+        } else {
+          std::memcpy(q + (std::size_t)(r * d + c) * 16, &S[r], 16);
+        }
+      }
+    }
+  }
+}
+
 template <class T>
-void execute(std::vector<C<T>>& s, const ketplan::Plan& plan, int n, int precision) {
+void execute(std::vector<C<T>>& s, ketplan::Plan plan, int n, int precision) {
+  run_builds(plan);
   for (const ketplan::Op& o : plan.ops) {
     switch (o.kind) {
       case ketplan::Kind::Pair:
@@ -524,6 +606,67 @@ int learned(const std::vector<ket_gate>& g, int n, int precision) {
 
 }  // namespace
 
+// Every fusion mode that builds block matrices (ket's Fuser, the block
+// packer at widths 2..6, the blocks-only packer), planned with the matrices
+// built on the host and as build records: after run_builds the two pools
+// must be equal byte for byte. Prints "ok build <records>".
+int builds(const std::vector<ket_gate>& g, int n, int precision) {
+  std::string err;
+  std::size_t records = 0;
+  const int modes[][2] = {{KET_FUSE_REFERENCE, 0}, {KET_FUSE_BLOCKS, 2}, {KET_FUSE_BLOCKS, 3},
+                          {KET_FUSE_BLOCKS, 4},    {KET_FUSE_BLOCKS, 5}, {KET_FUSE_BLOCKS, 6},
+                          {KET_FUSE_BLOCKS_ALL, 4}, {KET_FUSE_BLOCKS_ALL, 5}};
+  for (const auto& m : modes) {
+    ketplan::Plan p[2];
+    for (int dev = 0; dev < 2; ++dev) {
+      ketk::Tuning t = ketk::get_tuning(precision);
+      t.device_build = dev;
+      ketk::set_tuning(precision, t);
+      if (ketplan::plan_circuit(g.data(), g.size(), n, precision, m[0], m[1], &p[dev], &err)) {
+        std::fprintf(stderr, "plan: %s\n", err.c_str());
+        return 2;
+      }
+    }
+    if (!p[0].builds.empty()) {
+      std::printf("host-built plan has build records\n");
+      return 1;
+    }
+    records += p[1].builds.size();
+    run_builds(p[1]);
+    // the device plan's pool: the host plan's records, plus the build records
+    // and table after each block; compare each op's record
+    if (p[0].ops.size() != p[1].ops.size()) {
+      std::printf("plans differ in passes (mode %d k %d)\n", m[0], m[1]);
+      return 1;
+    }
+    for (std::size_t i = 0; i < p[0].ops.size(); ++i) {
+      const ketplan::Op &a = p[0].ops[i], &b = p[1].ops[i];
+      if (a.kind != b.kind) {
+        std::printf("pass %zu differs in kind\n", i);
+        return 1;
+      }
+      std::size_t bytes = 0;
+      if (a.kind == ketplan::Kind::MatrixExact) {
+        const std::size_t D = std::size_t{1} << a.k;
+        bytes = D * D * 16 + (a.k + D) * 8;
+      } else if (a.kind == ketplan::Kind::Block) {
+        ketk::BlockRecord h;
+        std::memcpy(&h, p[0].pool.data() + a.pool_offset, sizeof h);
+        const std::size_t D = std::size_t{1} << h.k;
+        bytes = h.mat_offset + D * D * (precision == 1 ? 8 : 16);
+      } else {
+        continue;
+      }
+      if (std::memcmp(p[0].pool.data() + a.pool_offset, p[1].pool.data() + b.pool_offset, bytes)) {
+        std::printf("record of pass %zu differs (mode %d k %d)\n", i, m[0], m[1]);
+        return 1;
+      }
+    }
+  }
+  std::printf("ok build %zu\n", records);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc != 5 && argc != 6) {
     std::fprintf(stderr, "usage: test_runs <records.bin> <n> <precision> <none|reference|runs|features|learned> [low bits]\n");
@@ -541,6 +684,7 @@ int main(int argc, char** argv) {
   }
   const std::string mode = argv[4];
   if (mode == "features") return features(g, n, precision);
+  if (mode == "build") return builds(g, n, precision);
   if (mode == "learned") return learned(g, n, precision);
   const bool reference = mode == "reference", runs = mode == "runs";
   return precision == 0 ? compare<double>(g, n, precision, reference, runs)

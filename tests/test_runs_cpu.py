@@ -105,3 +105,38 @@ def test_learned_planner_recording_features_match_full_packing(precision):
             r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), "features"],
                                capture_output=True, text=True, timeout=120)
             assert r.returncode == 0 and r.stdout.strip() == "ok features", r.stdout + r.stderr
+
+
+def every_gate_circuit(n: int, depth: int, seed: int):
+    """Random layers of all 24 gate types, angles uniform [0, 2pi)."""
+    import numpy as np
+    from ket_amd.gates import ARITY, NPARAMS, Circuit
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    names = sorted(ARITY)
+    for _ in range(depth):
+        for name in names:
+            qs = [int(q) for q in rng.choice(n, ARITY[name], replace=False)]
+            getattr(c, name)(*qs, *rng.uniform(0.0, 2 * np.pi, NPARAMS.get(name, 0)))
+    return c
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="ket_amd/test_runs not built")
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["every13", "every7", "qft14", "hea15", "mixed14"])
+def test_gpu_build_records_equal_host_matrices(name, precision):
+    """Block matrices left to the GPU (ket_tuning.device_build): the build
+    records, run through k_build's arithmetic on the CPU, must give block
+    records equal byte for byte to those built on the host, for ket's Fuser
+    and the block packer at widths 2..6 (every gate type, folded phase terms,
+    complex64 conversion)."""
+    c = {"every13": lambda: every_gate_circuit(13, 6, 11),
+         "every7": lambda: every_gate_circuit(7, 10, 12)}.get(name, CIRCUITS.get(name))()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c.bin")
+        c.records().tofile(path)
+        r = subprocess.run([EXE, path, str(c.n_qubits), str(precision), "build"],
+                           capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ok, what, records = r.stdout.split()
+    assert ok == "ok" and what == "build" and int(records) > 10
