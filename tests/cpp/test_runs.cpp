@@ -374,10 +374,13 @@ void run_builds(ketplan::Plan& plan) {
     std::memcpy(&h, plan.pool.data() + at, sizeof h);
     if (h.k < 1 || h.k > plan.build_kmax) bad_record("build width");
     std::vector<ketk::BuildOp> ops(h.n_ops);
-    std::memcpy(ops.data(), plan.pool.data() + at + sizeof h, ops.size() * sizeof(ketk::BuildOp));
+    if (!ops.empty())
+      std::memcpy(ops.data(), plan.pool.data() + at + sizeof h, ops.size() * sizeof(ketk::BuildOp));
     std::vector<ketk::BuildFold> folds(h.n_folds);
-    std::memcpy(folds.data(), plan.pool.data() + at + sizeof h + ops.size() * sizeof(ketk::BuildOp),
-                folds.size() * sizeof(ketk::BuildFold));
+    if (!folds.empty())
+      std::memcpy(folds.data(),
+                  plan.pool.data() + at + sizeof h + ops.size() * sizeof(ketk::BuildOp),
+                  folds.size() * sizeof(ketk::BuildFold));
     const uint32_t d = 1u << h.k;
     if (h.dest + (std::size_t)d * d * (h.f32 ? 8 : 16) > plan.pool.size()) bad_record("build dest");
     for (uint32_t c = 0; c < d; ++c) {
