@@ -304,7 +304,9 @@ def bench_form(args, circ, desc):
 def bench_hea(args):
     """Config 4: 4096 instances, widths U{10..22}, one sweep per step with a
     new parameter binding (two angle sets alternate), complex128 and complex64,
-    hipGraph replay; CPU baseline: ket::run + expval per instance."""
+    HBM instances as fused blocks and as packed exact runs (ket_tuning
+    batch_form), hipGraph replay; `value` is complex128 in the library's
+    default form. CPU baseline: ket::run + expval per instance."""
     sets = [hea_batch(angle_seed=a) for a in (0, 1)]
     ns = sets[0][0]
     gates = sum(c.n_unitary() for c in sets[0][1])
@@ -312,7 +314,11 @@ def bench_hea(args):
     log(f"[bench] hea4096: {len(ns)} instances, {gates} gates per sweep")
     out = {"metric": METRIC}
     res = {}
-    for label, prec in (("c128", N.C128), ("c64", N.C64)):
+    forms = [(f"{p}_{f}", prec, bf) for p, prec in (("c128", N.C128), ("c64", N.C64))
+             for f, bf in (("blocks", 0), ("runs", 1))]
+    default_form = {prec: N.get_tuning(prec)["batch_form"] for prec in (N.C128, N.C64)}
+    for label, prec, bf in forms:
+        N.set_tuning(prec, batch_form=bf)
         with N.Batch(ns, prec) as b:
             for w in range(max(args.warmup, 1)):
                 b.run_packed(packed[w % 2], block_k=0, use_graph=True)
@@ -333,11 +339,13 @@ def bench_hea(args):
                           "alg_GBps_device": st["alg_bytes"] / (st["device_ms"] / 1e3) / 1e9,
                           "launches": st["launches"],
                           "instances_lds": st["instances_lds"], "graph_replayed": st["graph_replayed"],
+                          "batch_form": "runs" if bf else "blocks",
                           "block_k": N.get_tuning(prec)["block_k"] or (4 if prec == N.C128 else 5)}
             log(f"[bench] {label}: {wall:.3f} s/sweep ({dev_ms / args.steps:.1f} ms device, "
                 f"{plan_s / args.steps:.3f} s planning of which {pack_s / args.steps:.3f} s "
                 f"packing, {up_s / args.steps:.3f} s upload), {len(ns) / wall:.0f} instances/s, "
                 f"{st['launches']} launches")
+        N.set_tuning(prec, batch_form=default_form[prec])
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -353,9 +361,10 @@ def bench_hea(args):
                    "sample": f"ket::run + expval(PauliSum) on 32 random instances of the sweep "
                              f"({frac_gates:.2%} of its gates), one process per instance, 1 thread"}
             log(f"[bench] cpu: {cpu['value']:.1f} instances/s")
+    main = res["c128_runs" if default_form[N.C128] else "c128_blocks"]
     out.update({
-        "value": res["c128"]["instances_per_s"], "unit": "instances/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["c128"]["s_per_sweep"] * 1e3,
+        "value": main["instances_per_s"], "unit": "instances/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main["s_per_sweep"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic",
         "config": {"workload": "hea4096: 4096 HEA instances (8 layers Ry,Rz + CZ ring), "

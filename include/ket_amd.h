@@ -344,6 +344,10 @@ typedef struct ket_tuning {
                                after the plan's upload, one thread per
                                column; 0 built on the host. Both use the same
                                products and sums and are equal bit for bit */
+  int32_t batch_form;       /* ket_batch_run's instances in HBM: 0 fused
+                               blocks (block_k), 1 packed exact runs (the run
+                               packer's reordered passes with fused
+                               multiply-adds, as KET_FUSE_RUNS) */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

@@ -6,10 +6,13 @@
 //   run whole in one workgroup each (k_batch_lds): one launch per width.
 // * Larger instances are grouped by width into back-to-back HBM arenas. Each
 //   instance is planned with blocks only (every pass a fused block), so
-//   instances of one width share their pass structure; one batched launch
-//   per pass then serves the whole group, each workgroup reading its own
-//   instance's matrix. A group whose instances plan differently falls back
-//   to one launch per instance and pass.
+//   instances of one width share their pass structure, or (ket_tuning
+//   batch_form = 1) as packed exact runs (every pass a run, fused
+//   multiply-adds), which share the launch sequence whenever their pass
+//   counts agree; one batched launch per pass then serves the whole group,
+//   each workgroup reading its own instance's record. A group whose
+//   instances plan differently falls back to one launch per instance and
+//   pass.
 // * With use_graph, the launch sequence is captured once in a hipGraph and
 //   replayed on later runs of the same structure; new angles only change
 //   the pool contents, which are uploaded before each replay.
@@ -60,7 +63,7 @@ namespace {
 int lds_max_qubits(int precision) { return precision == 0 ? 12 : 13; }
 
 struct Launch {
-  enum Kind { Lds, Fill, Block, BlockSingle, Expval } kind;
+  enum Kind { Lds, Fill, Block, BlockSingle, Run, RunSingle, Expval } kind;
   int n = 0, k = 0, n_terms = 0, max_qubits = 0;
   unsigned instances = 0;
   std::size_t arena_off = 0;   // amplitudes
@@ -71,7 +74,26 @@ struct Launch {
   std::size_t nq_off = 0, ops_off = 0, opoff_off = 0, z_off = 0, c_off = 0, toff_off = 0;
   int expval_terms = 0;
   int lowest = -1;             // Block, BlockSingle: the block's lowest qubit
+  bool contract = false;       // Run, RunSingle: fused multiply-adds
 };
+
+const ketk::RunRecord& run_record(const ketplan::Plan& p, const ketplan::Op& o) {
+  return *reinterpret_cast<const ketk::RunRecord*>(p.pool.data() + o.pool_offset);
+}
+
+// Instances planned as runs can share one launch per pass when every
+// instance has the same number of passes, all runs of one arithmetic form
+// (each workgroup reads its own instance's record, whatever its layout).
+bool same_run_launches(const ketplan::Plan& a, const ketplan::Plan& b) {
+  if (a.ops.size() != b.ops.size()) return false;
+  for (std::size_t i = 0; i < a.ops.size(); ++i) {
+    const ketplan::Op &x = a.ops[i], &y = b.ops[i];
+    if (x.kind != ketplan::Kind::Run || y.kind != ketplan::Kind::Run ||
+        x.run_contract != y.run_contract)
+      return false;
+  }
+  return true;
+}
 
 
 bool same_structure(const ketplan::Plan& a, const ketplan::Plan& b) {
@@ -180,10 +202,11 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
   for (std::size_t i : b->lds_inst) in_lds[i] = 1;
   std::vector<std::vector<ketk::LdsOp>> lds_ops;
   std::vector<ketplan::Plan> inst_plans;
+  const bool runs = ketk::get_tuning(P).batch_form == 1;
   {
     std::string err;
     if (int s2 = ketbatch::plan_instances(b->n, in_lds, gates, gate_off, P, block_k, &lds_ops,
-                                          &inst_plans, &err))
+                                          &inst_plans, &err, runs))
       return fail(s2, err);
   }
   const auto t_pack = std::chrono::steady_clock::now();
@@ -238,8 +261,11 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     bool uniform = true;
     for (std::size_t x = 0; x < inst.size(); ++x) {
       st.gates += plan_of(x).gates;
-      if (x && !same_structure(plan_of(0), plan_of(x))) uniform = false;
+      if (x && !(runs ? same_run_launches(plan_of(0), plan_of(x))
+                      : same_structure(plan_of(0), plan_of(x))))
+        uniform = false;
     }
+    if (runs && !same_run_launches(plan_of(0), plan_of(0))) uniform = false;  // a non-run pass
     const std::size_t base = b->group_base.at(n);
     Launch F{};
     F.kind = Launch::Fill;
@@ -250,7 +276,31 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
     key += "G" + std::to_string(n) + ":" + std::to_string(inst.size()) + ":" +
            (uniform ? "u" : "x") + ":";
     const ketplan::Plan& p0 = plan_of(0);
-    for (std::size_t op = 0; op < (uniform ? p0.ops.size() : 0); ++op) {
+    for (std::size_t op = 0; op < (uniform && runs ? p0.ops.size() : 0); ++op) {
+      // One run record per instance, at the largest record's stride.
+      std::size_t rec = 0;
+      for (std::size_t x = 0; x < inst.size(); ++x)
+        rec = std::max<std::size_t>(rec, run_record(plan_of(x), plan_of(x).ops[op]).bytes);
+      rec = align(rec, 256);
+      const std::size_t off = pool.reserve(rec * inst.size());
+      for (std::size_t x = 0; x < inst.size(); ++x) {
+        const ketplan::Op& o = plan_of(x).ops[op];
+        pool.copy_later(off + x * rec, plan_of(x).pool.data() + o.pool_offset,
+                        run_record(plan_of(x), o).bytes);
+      }
+      Launch L{};
+      L.kind = Launch::Run;
+      L.n = n;
+      L.instances = (unsigned)inst.size();
+      L.arena_off = base;
+      L.pool_off = off;
+      L.rec_stride = rec;
+      L.contract = p0.ops[op].run_contract;
+      launches.push_back(L);
+      key += "R" + std::to_string(rec) + (L.contract ? "c" : "e") + ",";
+      st.alg_bytes += 2.0 * B * (double)(uint64_t{1} << n) * inst.size();
+    }
+    for (std::size_t op = 0; op < (uniform && !runs ? p0.ops.size() : 0); ++op) {
       // One record per instance, same stride.
       const std::size_t r0 = p0.ops[op].pool_offset;
       const auto* h0 = reinterpret_cast<const ketk::BlockRecord*>(p0.pool.data() + r0);
@@ -279,7 +329,29 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
       // Each launch bakes in its instance's arena offset: the key marks
       // where every instance's passes start, so batches that split their
       // blocks differently among instances never share a captured graph.
-      for (std::size_t x = 0; x < inst.size(); ++x) {
+      for (std::size_t x = 0; x < inst.size() && runs; ++x) {
+        key += "|" + std::to_string(plan_of(x).ops.size()) + ":";
+        for (const ketplan::Op& o : plan_of(x).ops) {
+          if (o.kind != ketplan::Kind::Run) {
+            // runs-only plans hold runs only (every pass fits a tile)
+            return fail(KET_ERR_UNSUPPORTED, "ket_amd: a batched instance pass that is not a run");
+          }
+          const std::size_t bytes = run_record(plan_of(x), o).bytes;
+          const std::size_t off = pool.reserve(bytes);
+          pool.copy_later(off, plan_of(x).pool.data() + o.pool_offset, bytes);
+          Launch L{};
+          L.kind = Launch::RunSingle;
+          L.n = n;
+          L.instances = 1;
+          L.arena_off = base + (x << n);
+          L.pool_off = off;
+          L.contract = o.run_contract;
+          launches.push_back(L);
+          key += "r" + std::to_string(bytes) + (L.contract ? "c" : "e") + ",";
+          st.alg_bytes += 2.0 * B * (double)(uint64_t{1} << n);
+        }
+      }
+      for (std::size_t x = 0; x < inst.size() && !runs; ++x) {
         key += "|" + std::to_string(plan_of(x).ops.size()) + ":";
         for (const ketplan::Op& o : plan_of(x).ops) {
           const auto* h =
@@ -397,6 +469,13 @@ int ket_batch_run(ket_batch* b, const ket_gate* gates, const uint64_t* gate_off,
         case Launch::BlockSingle:
           e = ketk::launch_block(b->stream, P, arena, L.n, L.k, L.n_terms, pd + L.pool_off,
                                  L.lowest);
+          break;
+        case Launch::Run:
+          e = ketk::launch_run_batched(b->stream, P, arena, L.n, pd + L.pool_off, L.rec_stride,
+                                       L.instances, L.contract);
+          break;
+        case Launch::RunSingle:
+          e = ketk::launch_run(b->stream, P, arena, L.n, pd + L.pool_off, L.contract);
           break;
         case Launch::Expval:
           e = ketk::launch_expval_diag_batched(b->stream, P, arena, L.n, L.instances,

@@ -42,10 +42,31 @@ int plan_blocks_only(const ket_gate* g, std::size_t ng, int n, int precision, in
   return KET_OK;
 }
 
+int plan_runs_only(const ket_gate* g, std::size_t ng, int n, int precision,
+                   ketplan::Plan* plan, std::string* err) {
+  *plan = ketplan::Plan{};
+  std::vector<std::size_t> run;
+  for (std::size_t i = 0; i <= ng; ++i) {
+    if (i < ng) {
+      if (int st = ketplan::validate(g[i], n, err)) return st;
+    }
+    const bool unitary = i < ng && g[i].type >= KET_H && g[i].type <= KET_CSWAP;
+    if (unitary) {
+      ++plan->gates;
+      run.push_back(i);
+      continue;
+    }
+    if (!run.empty()) ketplan::pack_runs(g, run, n, precision, plan);
+    run.clear();
+  }
+  ketplan::group_runs(*plan, n, precision, /*contract=*/true, /*all=*/true);
+  return KET_OK;
+}
+
 int plan_instances(const std::vector<int>& n, const std::vector<char>& in_lds,
                    const ket_gate* gates, const uint64_t* gate_off, int precision, int block_k,
                    std::vector<std::vector<ketk::LdsOp>>* lds_ops,
-                   std::vector<ketplan::Plan>* plans, std::string* err) {
+                   std::vector<ketplan::Plan>* plans, std::string* err, bool runs) {
   const std::size_t ni = n.size();
   lds_ops->assign(ni, {});
   plans->assign(ni, {});
@@ -55,7 +76,8 @@ int plan_instances(const std::vector<int>& n, const std::vector<char>& in_lds,
     const ket_gate* g = gates + gate_off[i];
     const std::size_t ng = gate_off[i + 1] - gate_off[i];
     if (!in_lds[i]) {
-      status[i] = plan_blocks_only(g, ng, n[i], precision, block_k, &(*plans)[i], &errs[i]);
+      status[i] = runs ? plan_runs_only(g, ng, n[i], precision, &(*plans)[i], &errs[i])
+                       : plan_blocks_only(g, ng, n[i], precision, block_k, &(*plans)[i], &errs[i]);
       return;
     }
     for (std::size_t j = 0; j < ng; ++j) {

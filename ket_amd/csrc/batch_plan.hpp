@@ -75,13 +75,18 @@ ketk::LdsOp lds_op(const ketplan::Op& o);
 int plan_blocks_only(const ket_gate* g, std::size_t ng, int n, int precision, int block_k,
                      ketplan::Plan* plan, std::string* err);
 
+// Runs-only plan of one instance: the run packer's reordered passes, every
+// pass in an exact run with fused multiply-adds (KET_FUSE_RUNS's form).
+int plan_runs_only(const ket_gate* g, std::size_t ng, int n, int precision,
+                   ketplan::Plan* plan, std::string* err);
+
 // Every instance's host work, in parallel: the LDS ops of instances with
-// in_lds[i] set, the blocks-only plans of the others. Instance i's gates are
-// gates[gate_off[i] .. gate_off[i+1]). On failure returns the first failing
-// instance's status and message.
+// in_lds[i] set, the blocks-only (runs = false) or runs-only plans of the
+// others. Instance i's gates are gates[gate_off[i] .. gate_off[i+1]). On
+// failure returns the first failing instance's status and message.
 int plan_instances(const std::vector<int>& n, const std::vector<char>& in_lds,
                    const ket_gate* gates, const uint64_t* gate_off, int precision, int block_k,
                    std::vector<std::vector<ketk::LdsOp>>* lds_ops,
-                   std::vector<ketplan::Plan>* plans, std::string* err);
+                   std::vector<ketplan::Plan>* plans, std::string* err, bool runs = false);
 
 }  // namespace ketbatch

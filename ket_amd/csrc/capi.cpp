@@ -807,10 +807,12 @@ int check_tuning(const ket_tuning* t, ketk::Tuning* out) {
     return fail(KET_ERR_ARG, "ket_amd: run_low_bits must be 0 or in [3, 10]");
   if (t->device_build != 0 && t->device_build != 1)
     return fail(KET_ERR_ARG, "ket_amd: device_build must be 0 or 1");
+  if (t->batch_form != 0 && t->batch_form != 1)
+    return fail(KET_ERR_ARG, "ket_amd: batch_form must be 0 or 1");
   *out = {t->items_per_thread, t->nontemporal, t->blocks_per_cu,
           t->block_tiles ? t->block_tiles : 1, t->block_bpc ? t->block_bpc : 32,
           t->block_stage, t->pair_exchange, t->block_k, t->exact_runs, t->run_low_bits,
-          t->device_build};
+          t->device_build, t->batch_form};
   return KET_OK;
 }
 
@@ -826,6 +828,7 @@ void fill_tuning(const ketk::Tuning& k, ket_tuning* t) {
   t->exact_runs = k.exact_runs;
   t->run_low_bits = k.run_low_bits;
   t->device_build = k.device_build;
+  t->batch_form = k.batch_form;
 }
 }  // namespace
 

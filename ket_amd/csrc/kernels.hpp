@@ -42,6 +42,8 @@ struct Tuning {
   int run_low_bits;  // exact runs: low index bits kept in every tile
   int device_build;  // fused and exact block matrices built on the GPU (1)
                      // or on the host (0) from the same products
+  int batch_form;    // batched sweeps' HBM instances: 0 fused blocks, 1 packed
+                     // exact runs (KET_FUSE_RUNS)
 };
 // Per precision (0 = complex128, 1 = complex64).
 void set_tuning(int precision, const Tuning& t);
@@ -140,7 +142,7 @@ struct RunRecord {
   int32_t n_segs;
   uint16_t sbit[12];  // slot bits of tile bit b
   uint32_t phase_offset;  // bytes from the record start to the RunPhase table
-  int32_t pad;
+  uint32_t bytes;         // the whole record (with its matrices)
 };
 // A phase op as k_run_tile reads it: 32 bytes, so a group of four loads as
 // two scalar 64-byte loads. An entry with reg_mask 0 is a no-op.
@@ -171,6 +173,12 @@ inline int run_thread_bits(int precision) { return run_tile_bits(precision) - ru
 // equals the passes' within rounding only); false: bit for bit.
 hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
                       bool contract = false);
+// The same run on `instances` states of 2^n amplitudes back to back from s,
+// instance i reading its own record at rec_dev + i * rec_stride (records
+// may differ in everything but their kernel form: one grid per call,
+// blockIdx.y = instance).
+hipError_t launch_run_batched(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
+                              uint64_t rec_stride, unsigned instances, bool contract);
 
 // Block matrices built on the GPU (sv_build.hip). A build record is a
 // BuildHeader, n_ops BuildOps, n_folds BuildFolds; the plan's build table

@@ -466,7 +466,7 @@ namespace {
 void copy_build(Plan& plan, std::size_t from, std::size_t dest, bool move);
 }  // namespace
 
-void group_runs(Plan& plan, int n, int precision, bool contract) {
+void group_runs(Plan& plan, int n, int precision, bool contract, bool all) {
   const int tb = ketk::run_tile_bits(precision);
   if (n < tb) return;
   const int low = run_low_bits(precision);
@@ -489,7 +489,9 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
     // A run moves the whole state once; its passes alone move f sweeps.
     // Below 2^22 amplitudes the state sits in the caches and launches
     // dominate, so any two passes are worth a run there.
-    const bool worth = j - i >= 2 && (f > 1.2 || n <= 22);
+    // all: every pass that fits a tile runs as a run (one kernel form for
+    // every pass, as batched sweeps launch them).
+    const bool worth = all ? j > i : j - i >= 2 && (f > 1.2 || n <= 22);
     uint64_t P = 0;
     std::vector<ketk::RunOp> ops;
     std::vector<unsigned char> mats;
@@ -540,6 +542,7 @@ void group_runs(Plan& plan, int n, int precision, bool contract) {
     h.n_ops = (int)ops.size();
     h.n_segs = (int)segs.size();
     h.phase_offset = (uint32_t)phase_at;
+    h.bytes = (uint32_t)(head + mats.size());
     for (int b = 0; b < tb; ++b) h.sbit[b] = (uint16_t)ketk::run_slot(1u << b, row_bits);
     unsigned char* dst = plan.pool.data() + off;
     std::memcpy(dst, &h, sizeof h);

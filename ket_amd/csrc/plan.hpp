@@ -83,7 +83,7 @@ inline int default_block_k(int precision) {
 // Results are unchanged bit for bit, unless contract is set: then the runs
 // use fused multiply-adds (equal within rounding; for the modes that are
 // not bit-exact anyway). No-op below run_tile_bits qubits.
-void group_runs(Plan& plan, int n, int precision, bool contract = false);
+void group_runs(Plan& plan, int n, int precision, bool contract = false, bool all = false);
 
 // Lowest qubit of a MatrixExact or Block op.
 inline int lowest_qubit(const Op& o) {

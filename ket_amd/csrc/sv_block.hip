@@ -1220,12 +1220,12 @@ struct RunIO {
   }
 };
 
-// One workgroup per tile (grid-stride): load the tile, apply each segment's
-// ops in registers, store it back.
+// One workgroup per tile (grid-stride over tiles block, block + nblocks,
+// ...): load the tile, apply each segment's ops in registers, store it back.
 template <class T, int TB, int RB, bool FMA>
-__global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile(ketd::cplx<T>* __restrict__ s,
-                                                              uint64_t n_tiles,
-                                                              const unsigned char* __restrict__ rec) {
+__device__ __forceinline__ void run_tile_body(ketd::cplx<T>* __restrict__ s, uint64_t n_tiles,
+                                              const unsigned char* __restrict__ rec,
+                                              uint64_t block, uint64_t nblocks) {
   constexpr int RPT = 1 << RB, TBITS = TB - RB, NT = 1 << TBITS;
   static_assert(RPT <= kRunMaxRegs && TBITS <= kRunMaxThreadBits, "run layout");
   __shared__ ketd::cplx<T> L[1 << TB];
@@ -1235,9 +1235,7 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile(ketd::cplx<T>* __re
   const auto* phases = reinterpret_cast<const RunPhase*>(rec + h.phase_offset);
   RunIO<T, TB, NT> io;
   io.init(h, threadIdx.x);
-  // The next tile's loads are issued before this tile's stores, so both are
-  // in flight together.
-  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  for (uint64_t tile = block; tile < n_tiles; tile += nblocks) {
     typename RunIO<T, TB, NT>::V pv[RunIO<T, TB, NT>::S];
     const uint64_t fixed = deposit_outside(tile, h.pmask);
     // the thread's tile addresses are recomputed each tile rather than held
@@ -1286,6 +1284,23 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile(ketd::cplx<T>* __re
     io.store(s, fixed, L);
     __syncthreads();
   }
+}
+
+template <class T, int TB, int RB, bool FMA>
+__global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile(ketd::cplx<T>* __restrict__ s,
+                                                              uint64_t n_tiles,
+                                                              const unsigned char* __restrict__ rec) {
+  run_tile_body<T, TB, RB, FMA>(s, n_tiles, rec, blockIdx.x, gridDim.x);
+}
+
+// Batched: blockIdx.y = instance, its state 2^n amplitudes after the
+// previous one's, its record rec_stride bytes after the previous one's.
+template <class T, int TB, int RB, bool FMA>
+__global__ __launch_bounds__(1 << (TB - RB)) void k_run_tile_batched(
+    ketd::cplx<T>* __restrict__ s, uint64_t state_stride, uint64_t n_tiles,
+    const unsigned char* __restrict__ rec, uint64_t rec_stride) {
+  run_tile_body<T, TB, RB, FMA>(s + blockIdx.y * state_stride, n_tiles,
+                                rec + blockIdx.y * rec_stride, blockIdx.x, gridDim.x);
 }
 
 // The launch rule for vector blocks (see launch_block).
@@ -1353,6 +1368,40 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
               : block_vec<double, false>(st, s, groups, k, rec);
   return ph ? block_vec<float, true>(st, s, groups, k, rec)
             : block_vec<float, false>(st, s, groups, k, rec);
+}
+
+hipError_t launch_run_batched(hipStream_t st, int precision, void* s, int n, const void* rec_dev,
+                              uint64_t rec_stride, unsigned instances, bool contract) {
+  const int tb = run_tile_bits(precision);
+  if (n < tb || instances == 0 || instances > 65535 || (rec_stride % 16) != 0)
+    return hipErrorInvalidValue;
+  constexpr int TB0 = 11, RB0 = 3, TB1 = 12, RB1 = 4;
+  if (run_tile_bits(0) != TB0 || run_reg_bits(0) != RB0 || run_tile_bits(1) != TB1 ||
+      run_reg_bits(1) != RB1)
+    return hipErrorInvalidValue;
+  const uint64_t n_tiles = uint64_t{1} << (n - tb), stride = uint64_t{1} << n;
+  const Tuning t = get_tuning(precision);
+  // workgroups per instance: the grid of a single-state run, shared out
+  uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32) / instances;
+  if (cap < 1) cap = 1;
+  const dim3 grid((unsigned)(n_tiles < cap ? n_tiles : cap), instances);
+  const auto* rec = static_cast<const unsigned char*>(rec_dev);
+  constexpr unsigned NT0 = 1u << (TB0 - RB0), NT1 = 1u << (TB1 - RB1);
+  auto* d = static_cast<double2*>(s);
+  auto* f = static_cast<float2*>(s);
+  if (precision == 0 && contract)
+    k_run_tile_batched<double, TB0, RB0, true><<<grid, NT0, 0, st>>>(d, stride, n_tiles, rec,
+                                                                     rec_stride);
+  else if (precision == 0)
+    k_run_tile_batched<double, TB0, RB0, false><<<grid, NT0, 0, st>>>(d, stride, n_tiles, rec,
+                                                                      rec_stride);
+  else if (contract)
+    k_run_tile_batched<float, TB1, RB1, true><<<grid, NT1, 0, st>>>(f, stride, n_tiles, rec,
+                                                                    rec_stride);
+  else
+    k_run_tile_batched<float, TB1, RB1, false><<<grid, NT1, 0, st>>>(f, stride, n_tiles, rec,
+                                                                     rec_stride);
+  return hipGetLastError();
 }
 
 hipError_t launch_run(hipStream_t st, int precision, void* s, int n, const void* rec_dev,

@@ -19,7 +19,7 @@ namespace {
 // measured within 0.1%), with nontemporal tile transfers. Exact runs keep
 // 3 (complex128) / 4 (complex64) low bits in every tile. Block matrices
 // are built on the GPU.
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3, 1}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4, 1}};
+Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3, 1, 0}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4, 1, 0}};
 
 std::atomic<uint64_t> g_generation{1};
 

@@ -151,12 +151,12 @@ class Tuning(ctypes.Structure):
                 ("block_bpc", ctypes.c_int32), ("block_stage", ctypes.c_int32),
                 ("pair_exchange", ctypes.c_int32), ("block_k", ctypes.c_int32),
                 ("exact_runs", ctypes.c_int32), ("run_low_bits", ctypes.c_int32),
-                ("device_build", ctypes.c_int32)]
+                ("device_build", ctypes.c_int32), ("batch_form", ctypes.c_int32)]
 
 
 TUNING_FIELDS = ("items_per_thread", "nontemporal", "blocks_per_cu", "block_tiles", "block_bpc",
                  "block_stage", "pair_exchange", "block_k", "exact_runs", "run_low_bits",
-                 "device_build")
+                 "device_build", "batch_form")
 
 
 def get_tuning(precision: int = C128) -> dict:

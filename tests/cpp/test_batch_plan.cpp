@@ -110,12 +110,14 @@ int main() {
     }
     std::vector<char> in_lds(n.size());
     for (std::size_t i = 0; i < n.size(); ++i) in_lds[i] = n[i] <= (precision == 0 ? 12 : 13);
-    for (int block_k : {3, 4, 5}) {
+    // block widths 3, 4, 5 (fused blocks), then 0: the runs-only form
+    for (int block_k : {3, 4, 5, 0}) {
+      const bool runs = block_k == 0;
       std::vector<std::vector<ketk::LdsOp>> ops;
       std::vector<ketplan::Plan> plans;
       std::string err;
       if (int st = ketbatch::plan_instances(n, in_lds, gates.data(), off.data(), precision, block_k,
-                                            &ops, &plans, &err)) {
+                                            &ops, &plans, &err, runs)) {
         std::printf("plan_instances failed: %d %s\n", st, err.c_str());
         return 1;
       }
@@ -128,6 +130,17 @@ int main() {
           if (one.size() != ops[i].size() ||
               std::memcmp(one.data(), ops[i].data(), one.size() * sizeof(ketk::LdsOp)))
             ++bad;
+        } else if (runs) {
+          ketplan::Plan p;
+          if (ketbatch::plan_runs_only(gates.data() + off[i], off[i + 1] - off[i], n[i], precision,
+                                       &p, &err) ||
+              !same_plan(p, plans[i]))
+            ++bad;
+          for (const ketplan::Op& o : plans[i].ops)  // every pass a run, its size recorded
+            if (o.kind != ketplan::Kind::Run ||
+                reinterpret_cast<const ketk::RunRecord*>(plans[i].pool.data() + o.pool_offset)
+                        ->bytes == 0)
+              ++bad;
         } else {
           ketplan::Plan p;
           if (ketbatch::plan_blocks_only(gates.data() + off[i], off[i + 1] - off[i], n[i],

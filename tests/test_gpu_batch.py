@@ -152,3 +152,29 @@ def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
         assert np.array_equal(out[0], out[64])
     assert np.abs(out[0] - want).max() <= tol
     assert np.abs(out[64] - want).max() <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [(N.C128, 1e-10), (N.C64, 2e-4)])
+def test_batched_runs_match_oracle(precision, tol, restore_tuning):
+    """HBM instances as packed exact runs (ket_tuning.batch_form = 1): one
+    batched launch per run for instances of a width, each workgroup reading
+    its own instance's record; widths 13..22 and a non-uniform group (other
+    gate lists, so other run counts: one launch per instance and run), with
+    graph capture and replay on new angles."""
+    N.set_tuning(precision, batch_form=1)
+    ns = [13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 15, 22]
+    circs = [W.hea(n, 8, seed=700 + i) for i, n in enumerate(ns)]
+    circs[10] = W.random_mixed(15, 8, seed=3)  # width 15: instances plan differently
+    obs = [zz_obs(n) for n in ns]
+    want = np.array([oracle_value(c, o) for c, o in zip(circs, obs)])
+    with N.Batch(ns, precision) as b:
+        v1, s1 = b.run(circs, obs, block_k=0, use_graph=True)
+        assert s1["instances_hbm"] == len(ns) and s1["graph_replayed"] == 0
+        assert np.abs(v1 - want).max() <= tol
+        circs2 = [W.hea(n, 8, seed=900 + i) if i != 10 else circs[10] for i, n in enumerate(ns)]
+        want2 = np.array([oracle_value(c, o) for c, o in zip(circs2, obs)])
+        v2, s2 = b.run(circs2, obs, block_k=0, use_graph=True)
+        v3, s3 = b.run(circs2, obs, block_k=0, use_graph=False)
+    assert s2["graph_replayed"] == 1
+    assert np.abs(v2 - want2).max() <= tol and np.abs(v3 - want2).max() <= tol
+    assert np.array_equal(v2, v3)  # the replay gives the eager launches' bits
