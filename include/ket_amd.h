@@ -347,7 +347,8 @@ typedef struct ket_tuning {
   int32_t batch_form;       /* ket_batch_run's instances in HBM: 0 fused
                                blocks (block_k), 1 packed exact runs (the run
                                packer's reordered passes with fused
-                               multiply-adds, as KET_FUSE_RUNS) */
+                               multiply-adds, as KET_FUSE_RUNS); default 1
+                               complex128, 0 complex64 */
 } ket_tuning;
 /* ket_set_tuning sets both precisions; ket_get_tuning reads complex128's.
  * The _for forms address one precision (KET_C128 or KET_C64). */

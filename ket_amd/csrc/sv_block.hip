@@ -1057,9 +1057,13 @@ __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double
   constexpr int D = 1 << K;
 #pragma unroll
   for (int g = 0; g < RPT / D; ++g) {
-    // a compiler-only fence: keeps the matrix reads (scalar loads) with their
-    // group rather than held in registers across groups
-    __asm__ volatile("" ::: "memory");
+    // an offset of zero that the compiler cannot see: each group reads (and
+    // converts) the matrix again rather than holding all D^2 entries in
+    // registers across groups (the loads are invariant, so a memory fence
+    // alone does not keep them apart)
+    int zero = 0;
+    __asm__ volatile("" : "+s"(zero));
+    const double2* mg = mat + zero;
     ketd::cplx<T> in[D];
 #pragma unroll
     for (int e = 0; e < D; ++e) in[e] = v[g * D + e];
@@ -1068,7 +1072,7 @@ __device__ __forceinline__ void reg_matrix(ketd::cplx<T> (&v)[RPT], const double
       ketd::cplx<T> acc = ketd::mk<T>(T(0), T(0));
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        acc = rmac<T, FMA>(acc, ketd::mk<T>((T)mat[r * D + c].x, (T)mat[r * D + c].y), in[c]);
+        acc = rmac<T, FMA>(acc, ketd::mk<T>((T)mg[r * D + c].x, (T)mg[r * D + c].y), in[c]);
       v[g * D + r] = acc;
     }
   }
@@ -1083,6 +1087,33 @@ __device__ __forceinline__ double uniform_f64(double x) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// a *= ph in place with ket's complex product: the four products, each
+// rounded, into temporaries, then x = xx - yy and y = xy + yx into a's own
+// registers. (Compiled, the real sum went to a fresh register and was
+// copied back: one more VALU instruction per amplitude and phase.)
+__device__ __forceinline__ void phase_mul(ketd::cplx<double>& a, ketd::cplx<double> ph) {
+  double xx, yy, xy;
+  __asm__("v_mul_f64 %2, %0, %5\n\t"
+          "v_mul_f64 %3, %1, %6\n\t"
+          "v_mul_f64 %4, %0, %6\n\t"
+          "v_mul_f64 %1, %1, %5\n\t"
+          "v_add_f64 %0, %2, -%3\n\t"
+          "v_add_f64 %1, %4, %1"
+          : "+v"(a.x), "+v"(a.y), "=&v"(xx), "=&v"(yy), "=&v"(xy)
+          : "s"(ph.x), "s"(ph.y));
+}
+__device__ __forceinline__ void phase_mul(ketd::cplx<float>& a, ketd::cplx<float> ph) {
+  float xx, yy, xy;
+  __asm__("v_mul_f32 %2, %0, %5\n\t"
+          "v_mul_f32 %3, %1, %6\n\t"
+          "v_mul_f32 %4, %0, %6\n\t"
+          "v_mul_f32 %1, %1, %5\n\t"
+          "v_sub_f32 %0, %2, %3\n\t"
+          "v_add_f32 %1, %4, %1"
+          : "+v"(a.x), "+v"(a.y), "=&v"(xx), "=&v"(yy), "=&v"(xy)
+          : "s"(ph.x), "s"(ph.y));
+}
+
 // A stretch of `count` consecutive phase ops (tab: their RunPhase entries)
 // on the thread's registers. Whole tiles missing an op's outside bits skip
 // it and registers missing its register condition are skipped (uniform
@@ -1090,12 +1121,12 @@ __device__ __forceinline__ double uniform_f64(double x) {
 // update is a branch around it alone, so the registers stay in place from op
 // to op (a branch tree over the condition with straight-line leaves made the
 // compiler merge the leaves into one block that indexes the registers at run
-// time, through scratch memory); the four products are taken before either
-// sum. base = the tile-index bits of the thread's lanes, fixed = the tile's
-// bits outside P. Entries load four at a time as 64-bit words: each
-// stretch's entries are padded with no-op entries (register mask 0) to a
-// multiple of four, so a group never reaches past its stretch and needs no
-// count test; two scalar loads and one wait per four ops.
+// time, through scratch memory); each update is phase_mul. base = the
+// tile-index bits of the thread's lanes, fixed = the tile's bits outside P.
+// Entries load four at a time as 64-bit words: each stretch's entries are
+// padded with no-op entries (register mask 0) to a multiple of four, so a
+// group never reaches past its stretch and needs no count test; two scalar
+// loads and one wait per four ops.
 template <class T, int RPT>
 __device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunPhase* tab, int count,
                                            uint32_t base, uint64_t fixed) {
@@ -1124,15 +1155,8 @@ __device__ __forceinline__ void reg_phases(ketd::cplx<T> (&v)[RPT], const RunPha
       if ((base & lane[i]) == lane[i]) {  // per lane (lane 0: every lane)
 #pragma unroll
         for (int j = 0; j < RPT; ++j)
-          if (__builtin_expect((rm[i] >> j) & 1, 1)) {  // uniform
-            const T xy = v[j].x * ph[i].y, yx = v[j].y * ph[i].x;
-            const T xx = v[j].x * ph[i].x, yy = v[j].y * ph[i].y;
-            const T nx = xx - yy;
-            // the real part first: it can overwrite the amplitude's own
-            // registers, and so can the imaginary part after it
-            __builtin_amdgcn_sched_barrier(0);
-            v[j] = ketd::mk<T>(nx, xy + yx);
-          }
+          if (__builtin_expect((rm[i] >> j) & 1, 1))  // uniform
+            phase_mul(v[j], ph[i]);
       }
     }
   }

@@ -20,8 +20,18 @@ def zz_obs(n):
     return [(1.0, p) for p in W.zz_chain(n)] if n > 1 else [(1.0, "Z")]
 
 
+@pytest.fixture
+def restore_tuning():
+    saved = {p: N.get_tuning(p) for p in (N.C128, N.C64)}
+    yield
+    for p, t in saved.items():
+        N.set_tuning(p, **t)
+
+
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("precision,tol", [(N.C128, 1e-10), (N.C64, 2e-4)])
-def test_hea_sweep_matches_oracle(precision, tol):
+def test_hea_sweep_matches_oracle(precision, tol, form, restore_tuning):
+    N.set_tuning(precision, batch_form=form)  # HBM instances as blocks / as runs
     rng = np.random.default_rng(0)
     ns = [int(x) for x in rng.integers(2, 17, 40)] + [1, 12, 13, 14]
     circs = [W.hea(n, 8, seed=100 + i) for i, n in enumerate(ns)]
@@ -89,11 +99,12 @@ def blocks_on(n, groups, seed):
     return c
 
 
-def test_graph_replay_follows_instance_boundaries():
+def test_graph_replay_follows_instance_boundaries(restore_tuning):
     """Two same-width instances whose passes split 3 + 2 and then 2 + 3: the
     launch counts, pool size and per-pass shapes agree, but each launch
     addresses its own instance's state, so the second sweep must not replay
     the first sweep's graph."""
+    N.set_tuning(N.C128, batch_form=0)  # fused blocks
     n = 14
     three = [range(0, 4), range(4, 8), range(8, 12)]
     two = [range(0, 4), range(6, 10)]
@@ -120,14 +131,6 @@ def test_batch_rejects_bad_input():
             b.run([c], [[(1.0, "XII")]])
 
 
-@pytest.fixture
-def restore_tuning():
-    saved = {p: N.get_tuning(p) for p in (N.C128, N.C64)}
-    yield
-    for p, t in saved.items():
-        N.set_tuning(p, **t)
-
-
 @pytest.mark.parametrize("precision", [N.C128, N.C64])
 @pytest.mark.parametrize("k", [3, 4, 5])
 def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
@@ -140,6 +143,7 @@ def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
     circs = [W.hea(n, 4, seed=900 + i) for i, n in enumerate(ns)]
     obs = [zz_obs(n) for n in ns]
     out = {}
+    N.set_tuning(precision, batch_form=0)  # fused blocks
     for stage in (0, 64):
         N.set_tuning(block_stage=stage)
         with N.Batch(ns, precision) as b:
@@ -158,20 +162,21 @@ def test_batched_lds_tiles_equal_direct(precision, k, restore_tuning):
 def test_batched_runs_match_oracle(precision, tol, restore_tuning):
     """HBM instances as packed exact runs (ket_tuning.batch_form = 1): one
     batched launch per run for instances of a width, each workgroup reading
-    its own instance's record; widths 13..22 and a non-uniform group (other
-    gate lists, so other run counts: one launch per instance and run), with
-    graph capture and replay on new angles."""
+    its own instance's record; widths 14..22 (above the LDS-resident widths
+    of both precisions) and a non-uniform group (other gate lists, so other
+    run counts: one launch per instance and run), with graph capture and
+    replay on new angles."""
     N.set_tuning(precision, batch_form=1)
-    ns = [13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 15, 22]
+    ns = [14, 15, 16, 17, 18, 19, 20, 21, 22, 15, 22]
     circs = [W.hea(n, 8, seed=700 + i) for i, n in enumerate(ns)]
-    circs[10] = W.random_mixed(15, 8, seed=3)  # width 15: instances plan differently
+    circs[9] = W.random_mixed(15, 8, seed=3)  # width 15: instances plan differently
     obs = [zz_obs(n) for n in ns]
     want = np.array([oracle_value(c, o) for c, o in zip(circs, obs)])
     with N.Batch(ns, precision) as b:
         v1, s1 = b.run(circs, obs, block_k=0, use_graph=True)
         assert s1["instances_hbm"] == len(ns) and s1["graph_replayed"] == 0
         assert np.abs(v1 - want).max() <= tol
-        circs2 = [W.hea(n, 8, seed=900 + i) if i != 10 else circs[10] for i, n in enumerate(ns)]
+        circs2 = [W.hea(n, 8, seed=900 + i) if i != 9 else circs[9] for i, n in enumerate(ns)]
         want2 = np.array([oracle_value(c, o) for c, o in zip(circs2, obs)])
         v2, s2 = b.run(circs2, obs, block_k=0, use_graph=True)
         v3, s3 = b.run(circs2, obs, block_k=0, use_graph=False)
