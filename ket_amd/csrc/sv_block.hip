@@ -592,18 +592,26 @@ __device__ __forceinline__ void block_tile_body(ketd::cplx<T>* __restrict__ s, u
   }
 }
 
-// complex64 MFMA tiles: 256 threads, TB = 12; each wave applies the block
-// to TPW tiles of 16 groups with the operand layout of k_block_mfma.
+// complex64 MFMA tiles: 256 threads, tiles of 2^TB amplitudes
+// (mfma_tile_bits); each wave applies the block to TPW tiles of 16 groups
+// with the operand layout of k_block_mfma. Tile sizes by measurement on
+// QFT-30: k = 4 in 2^11-amplitude tiles (112 VGPRs, four waves per SIMD
+// instead of two; the step 78.2 -> 68.8 ms, the phase-heavy passes
+// 5.5-7.0 -> 3.7-4.4 ms), k = 5 in 2^12 (2^11 measured equal: 60.3 against
+// 60.5 ms, and 1% slower on 33-qubit brickwork).
+constexpr int kMfmaTB4 = 11, kMfmaTB5 = 12;
+constexpr int mfma_tile_bits(int k) { return k == 4 ? kMfmaTB4 : kMfmaTB5; }
 template <int K, bool PH>
 __device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uint64_t n_tiles,
                                                      const unsigned char* __restrict__ rec,
                                                      uint64_t block, uint64_t nblocks) {
-  constexpr int TB = 12;
+  constexpr int TB = mfma_tile_bits(K);
   constexpr int D = 1 << K;
   constexpr int RB = 2 * D / 16;
   constexpr int KS = 2 * D / 4;
   constexpr int NE = D / 4;
   constexpr int TPW = (1 << (TB - K)) / 16 / (kThreads / 64);
+  static_assert(TPW >= 1 && (1 << (TB - K)) % 64 == 0 && TB - K >= 4, "MFMA tile shape");
   using IO = TileIO<float, TB, kThreads>;
   constexpr int G = 1 << (TB - K);  // groups per tile
   __shared__ float2 L[1 << TB];
@@ -824,7 +832,7 @@ hipError_t block_tile(hipStream_t st, void* s, int n, int k, const unsigned char
 template <bool PH>
 hipError_t block_tile_mfma(hipStream_t st, void* s, int n, int k, const unsigned char* rec) {
   const Tuning t = get_tuning(1);
-  const uint64_t tiles = uint64_t{1} << (n - 12);
+  const uint64_t tiles = uint64_t{1} << (n - mfma_tile_bits(k));
   const uint64_t cap = 256 * (uint64_t)(t.block_bpc > 0 ? t.block_bpc : 32);
   const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
   auto* p = static_cast<float2*>(s);
@@ -855,7 +863,7 @@ template <bool PH>
 hipError_t block_tile_mfma_batched(hipStream_t st, void* s, uint64_t state_stride, int n, int k,
                                    const unsigned char* rec, uint64_t rec_stride,
                                    unsigned instances) {
-  const uint64_t tiles = uint64_t{1} << (n - 12);
+  const uint64_t tiles = uint64_t{1} << (n - mfma_tile_bits(k));
   const dim3 grid((unsigned)(tiles < 64 ? tiles : 64), instances);
   auto* p = static_cast<float2*>(s);
   if (k == 4)
@@ -1348,7 +1356,7 @@ hipError_t launch_block_batched(hipStream_t st, int precision, void* s, int n, i
   const bool ph = n_terms > 0;
   // The same choice as launch_block: MFMA tiles for complex64 k = 4, 5; LDS
   // tiles for other widths 3..5 whose lowest qubit is below block_stage.
-  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= 12)
+  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= mfma_tile_bits(k))
     return ph ? block_tile_mfma_batched<true>(st, s, dim, n, k, rec, rec_stride, instances)
               : block_tile_mfma_batched<false>(st, s, dim, n, k, rec, rec_stride, instances);
   if (vector_tiles(t, precision, k, n_terms, lowest) && n >= 11) {
@@ -1377,7 +1385,7 @@ hipError_t launch_block(hipStream_t st, int precision, void* s, int n, int k,
   // trailing phase terms, only when it is inside a 128-byte line, where the
   // direct kernel's loads do not coalesce (elsewhere the direct kernel,
   // with one more wave per SIMD, measured 2-16% faster on phase blocks).
-  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= 12)
+  if (precision == 1 && (k == 4 || k == 5) && t.block_stage > 0 && n >= mfma_tile_bits(k))
     return ph ? block_tile_mfma<true>(st, s, n, k, rec) : block_tile_mfma<false>(st, s, n, k, rec);
   if (precision == 1 && (k == 4 || k == 5) && groups >= 16)
     return ph ? block_mfma<true>(st, s, groups, k, rec) : block_mfma<false>(st, s, groups, k, rec);
