@@ -8,16 +8,20 @@
 // a group g (fixed out-of-Q bits) C and A_j are sums of term angles whose
 // out-of-Q bits are all set in g, so they are evaluated once per group.
 //
-// complex64, k in {4, 5}: MFMA. M is held in its real form
-//   [[Re M, -Im M], [Im M, Re M]]  (2D x 2D, D = 2^k)
-// as v_mfma_f32_16x16x4f32 A-operand registers for the whole kernel; each
-// wave streams tiles of 16 groups (B operand: real and imaginary parts of
-// the inputs, 4 per lane per k-step) and each lane finishes D/4 outputs.
+// complex64, k in {4, 5}: MFMA. The block's matrix is held as
+// v_mfma_f32_16x16x4f32 A-operand registers for the whole kernel: for k = 5
+// Re M, Im M and Re M + Im M (D x D each, D = 2^k; three real products per
+// block, Gauss's form: MfmaGauss), for k = 4 the real form [[Re M, -Im M],
+// [Im M, Re M]] (MfmaReal); each wave streams tiles of 16 groups (B operand:
+// parts of the inputs, one per lane per k-step) and each lane finishes D/4
+// outputs.
 // Every other case: one thread per group, the matrix broadcast from LDS,
 // rows accumulated with fma().
 //
 // Pool record (written by the planner, 256-byte aligned):
 //   BlockRecord header | term masks (uint64) | term angles (double) | matrix
+#include <type_traits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -187,14 +191,107 @@ __global__ __launch_bounds__(kThreads) void k_block_vec_batched(
 }
 
 // ---- MFMA path (complex64, k = 4 or 5) ----
+// A block on 16 groups as three real D x D products (Gauss's form of the
+// complex product): T1 = Re M . Re x, T2 = Im M . Im x, T3 = (Re M + Im M) .
+// (Re x + Im x), then Re y = T1 - T2, Im y = (T3 - T1) - T2. 3 D^2 real
+// multiply-adds per group instead of the 4 D^2 of the real form (MfmaReal):
+// k = 5 takes 48 v_mfma_f32_16x16x4f32 per 16 groups instead of 64 (QFT-30
+// complex64 k = 5 60.5 -> 58.6 ms, 33-qubit brickwork 3.82 -> 3.73 s per
+// step). Operand layout of the 16x16x4
+// step: lane (col, q) holds A[row = rb*16 + col][kk = ks*4 + q] and
+// B[kk = ks*4 + q][group col]; the result T[rb][v] is row rb*16 + 4q + v.
+template <int K>
+struct MfmaGauss {
+  static constexpr int D = 1 << K;
+  static constexpr int RB = D / 16;  // 16-row blocks
+  static constexpr int KS = D / 4;   // 4-deep k-steps = inputs per lane
+  float a[3][RB][KS];
+  __device__ __forceinline__ void load(const float2* __restrict__ M, int col, int q) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const float2 m = M[(rb * 16 + col) * D + ks * 4 + q];
+        a[0][rb][ks] = m.x;
+        a[1][rb][ks] = m.y;
+        a[2][rb][ks] = m.x + m.y;
+      }
+  }
+  // in[i]: element q + 4i of the lane's group; out[rb][v]: element
+  // rb*16 + 4q + v of the result.
+  __device__ __forceinline__ void apply(const float2 (&in)[KS], float2 (&out)[RB][4]) const {
+    floatx4 t[3][RB];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) t[p][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const float b[3] = {in[ks].x, in[ks].y, in[ks].x + in[ks].y};
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          t[p][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][rb][ks], b[p], t[p][rb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        out[rb][v] = make_float2(t[0][rb][v] - t[1][rb][v], (t[2][rb][v] - t[0][rb][v]) - t[1][rb][v]);
+  }
+};
+
+// The same block in the real form [[Re M, -Im M], [Im M, Re M]] (2D x 2D):
+// 4 D^2 multiply-adds per group, but no sums of inputs or of products
+// around the MFMA. k = 4 tiles are bound by their memory traffic, not by
+// the MFMA (Gauss's form measured 3% slower there), so k = 4 keeps this form.
+template <int K>
+struct MfmaReal {
+  static constexpr int D = 1 << K;
+  static constexpr int RB = D / 16;      // 16-row blocks of the outputs
+  static constexpr int KS = D / 4;       // inputs per lane
+  float a[2 * RB][2 * KS];               // [row block][k-step] of the real form
+  __device__ __forceinline__ void load(const float2* __restrict__ M, int col, int q) {
+#pragma unroll
+    for (int rb = 0; rb < 2 * RB; ++rb)
+#pragma unroll
+      for (int ks = 0; ks < 2 * KS; ++ks) {
+        const int row = rb * 16 + col, c = ks * 4 + q;
+        const float2 m = M[(row % D) * D + (c % D)];
+        const bool im_r = row >= D, im_c = c >= D;
+        a[rb][ks] = !im_r ? (!im_c ? m.x : -m.y) : (!im_c ? m.y : m.x);
+      }
+  }
+  __device__ __forceinline__ void apply(const float2 (&in)[KS], float2 (&out)[RB][4]) const {
+    floatx4 t[2 * RB];
+#pragma unroll
+    for (int rb = 0; rb < 2 * RB; ++rb) t[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2 * KS; ++ks) {
+      const float b = ks < KS ? in[ks].x : in[ks - KS].y;
+#pragma unroll
+      for (int rb = 0; rb < 2 * RB; ++rb)
+        t[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, t[rb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[rb][v] = make_float2(t[rb][v], t[rb + RB][v]);
+  }
+};
+// The MFMA form of a k-qubit block (both MFMA kernels use it, so their
+// results agree bit for bit).
+template <int K>
+using MfmaBlock = std::conditional_t<K == 5, MfmaGauss<K>, MfmaReal<K>>;
+
 template <int K, bool PH, int TPI>
 __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
                                                           uint64_t n_groups,
                                                           const unsigned char* __restrict__ rec) {
   constexpr int D = 1 << K;
-  constexpr int RB = 2 * D / 16;  // 16-row blocks of the real form
-  constexpr int KS = 2 * D / 4;   // 4-deep k-steps
-  constexpr int NE = D / 4;       // inputs and outputs per lane per tile
+  constexpr int RB = D / 16;  // 16-row blocks of the outputs
+  constexpr int NE = D / 4;   // inputs per lane per tile
   __shared__ uint64_t off[D];
   __shared__ uint64_t tm[PH ? kMaxTerms : 1];
   __shared__ double th[PH ? kMaxTerms : 1];
@@ -216,18 +313,8 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
   const int lane = threadIdx.x & 63;
   const int col = lane & 15;  // group within the tile (B/D column)
   const int q = lane >> 4;    // k index within a k-step (A column / B row)
-  // A operand: lane holds A[row = rb*16 + col][kk = ks*4 + q] of the real form.
-  const auto* M = reinterpret_cast<const float2*>(rec + h.mat_offset);
-  float a[RB][KS];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int row = rb * 16 + col, c = ks * 4 + q;
-      const float2 m = M[(row % D) * D + (c % D)];
-      const bool im_r = row >= D, im_c = c >= D;
-      a[rb][ks] = !im_r ? (!im_c ? m.x : -m.y) : (!im_c ? m.y : m.x);
-    }
+  MfmaBlock<K> mm;
+  mm.load(reinterpret_cast<const float2*>(rec + h.mat_offset), col, q);
 
   const uint64_t n_tiles = n_groups / 16;
   const uint64_t wave = (uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -247,16 +334,8 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
 #pragma unroll
     for (int u = 0; u < TPI; ++u) {
       if (u > 0 && tile0 + u >= n_tiles) break;
-      floatx4 acc[RB];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) acc[rb] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const float b = ks < NE ? in[u][ks].x : in[u][ks - NE].y;
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[rb], 0, 0, 0);
-      }
+      float2 y[RB][4];
+      mm.apply(in[u], y);
       PhaseFactors<float, K> ph;
       float2 G = make_float2(1.f, 0.f);
       if constexpr (PH) {
@@ -278,11 +357,11 @@ __global__ __launch_bounds__(kThreads) void k_block_mfma(float2* __restrict__ s,
         if (q & 2) G = ketd::cmul<float>(G, ph.F[3]);
       }
 #pragma unroll
-      for (int rb = 0; rb < RB / 2; ++rb)
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int e = rb * 16 + 4 * q + v;
-          float2 o = make_float2(acc[rb][v], acc[rb + RB / 2][v]);
+          float2 o = y[rb][v];
           if constexpr (PH) {
             float2 p = G;
             if (v & 1) p = ketd::cmul<float>(p, ph.F[0]);
@@ -607,9 +686,8 @@ __device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uin
                                                      uint64_t block, uint64_t nblocks) {
   constexpr int TB = mfma_tile_bits(K);
   constexpr int D = 1 << K;
-  constexpr int RB = 2 * D / 16;
-  constexpr int KS = 2 * D / 4;
-  constexpr int NE = D / 4;
+  constexpr int RB = D / 16;  // 16-row blocks of the outputs
+  constexpr int NE = D / 4;   // inputs per lane per tile
   constexpr int TPW = (1 << (TB - K)) / 16 / (kThreads / 64);
   static_assert(TPW >= 1 && (1 << (TB - K)) % 64 == 0 && TB - K >= 4, "MFMA tile shape");
   using IO = TileIO<float, TB, kThreads>;
@@ -636,17 +714,8 @@ __device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uin
   const int col = lane & 15;
   const int q = lane >> 4;
   const int wave = threadIdx.x >> 6;
-  const auto* M = reinterpret_cast<const float2*>(rec + h.mat_offset);
-  float a[RB][KS];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int row = rb * 16 + col, c = ks * 4 + q;
-      const float2 m = M[(row % D) * D + (c % D)];
-      const bool im_r = row >= D, im_c = c >= D;
-      a[rb][ks] = !im_r ? (!im_c ? m.x : -m.y) : (!im_c ? m.y : m.x);
-    }
+  MfmaBlock<K> mm;
+  mm.load(reinterpret_cast<const float2*>(rec + h.mat_offset), col, q);
   uint32_t lg[TPW];
 #pragma unroll
   for (int u = 0; u < TPW; ++u)
@@ -679,21 +748,13 @@ __device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uin
     }
     io.stash(L, v);
     __syncthreads();
-    floatx4 acc[TPW][RB];
+    float2 y[TPW][RB][4];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
       float2 in[NE];
 #pragma unroll
       for (int i = 0; i < NE; ++i) in[i] = L[lg[u] ^ oq_in ^ qoff<K>(4 * i, qbit)];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) acc[u][rb] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const float b = ks < NE ? in[ks].x : in[ks - NE].y;
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-          acc[u][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][ks], b, acc[u][rb], 0, 0, 0);
-      }
+      mm.apply(in, y[u]);
     }
     // The four lanes of a group read and write overlapping slots.
     __syncthreads();
@@ -712,10 +773,10 @@ __device__ __forceinline__ void block_tile_mfma_body(float2* __restrict__ s, uin
         if (q & 2) Gq = ketd::cmul<float>(Gq, F[3]);
       }
 #pragma unroll
-      for (int rb = 0; rb < RB / 2; ++rb)
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float2 o = make_float2(acc[u][rb][v], acc[u][rb + RB / 2][v]);
+          float2 o = y[u][rb][v];
           if constexpr (PH) {
             float2 p = Gq;
             if (v & 1) p = ketd::cmul<float>(p, F[0]);
