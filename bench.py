@@ -434,13 +434,15 @@ def bench_planner(args):
                every form (greedy blocks, ket's Fuser as exact runs, the
                run packer, learned), complex64 -> eval_<i>.json;
       report:  merge the eval_*.json into the result line;
+      cases:   QFT-30, 30-qubit brickwork and one 30-qubit random circuit
+               in every form and learned (where the forms trade places);
       all:     collect, train, eval and report in this process."""
     from ket_amd import costmodel as CM
     out = {"metric": METRIC}
     ddir = os.path.abspath(args.data_dir)
     os.makedirs(ddir, exist_ok=True)
     shard = lambda xs: xs[args.shard::args.shards]
-    stages = ("collect", "train", "eval", "report") if args.stage == "all" else (args.stage,)
+    stages = ("collect", "train", "eval", "report", "cases") if args.stage == "all" else (args.stage,)
     states = CM.States(N.C64)
     mlp = None
     try:
@@ -450,6 +452,13 @@ def bench_planner(args):
                 f"(shard {args.shard}/{args.shards})")
             t0 = time.perf_counter()
             X, y = CM.collect(circs, states, seed=1 + args.shard, log=log)
+            if args.ladder_circuits:
+                lad = shard(CM.ladder_circuits(args.ladder_circuits, seed=4321))
+                log(f"[bench] planner: timing windows of {len(lad)} QFT-like circuits")
+                X2, y2 = CM.collect(lad, states, long_len=args.ladder_window, seed=101 + args.shard,
+                                    log=log, long_forms=CM.LADDER_FORMS)
+                X, y = np.concatenate([X, X2]), np.concatenate([y, y2])
+                circs = circs + lad
             np.savez(os.path.join(ddir, f"data_{args.shard}.npz"), X=X, y=y,
                      circuits=len(circs), seconds=time.perf_counter() - t0)
             log(f"[bench] {len(y)} timed passes in {time.perf_counter() - t0:.0f} s")
@@ -463,7 +472,11 @@ def bench_planner(args):
             n_circ = int(sum(int(p["circuits"]) for p in parts))
             log(f"[bench] planner: training on {len(y)} passes of {n_circ} circuits")
             t0 = time.perf_counter()
-            mlp, hist = CM.train(X, y, epochs=60, log=log)
+            init = None
+            if args.init_model:
+                init = np.load(args.init_model, allow_pickle=False)["params"]
+                log(f"[bench] planner: fine-tuning {args.init_model}")
+            mlp, hist = CM.train(X, y, epochs=args.epochs, lr=args.lr, log=log, init=init)
             t_train = time.perf_counter() - t0
             dest = os.path.abspath(args.model_out)
             os.makedirs(os.path.dirname(dest), exist_ok=True)
@@ -482,6 +495,14 @@ def bench_planner(args):
             recs = CM.time_forms(mlp, held, states, reps=3, log=log)
             with open(os.path.join(ddir, f"eval_{args.shard}.json"), "w") as f:
                 json.dump(recs, f)
+        if "cases" in stages:
+            # named circuits where the forms trade places: QFT-30 (blocks
+            # fastest), 30-qubit brickwork and a config-5 circuit (runs)
+            if mlp is None:
+                mlp = CM.load()
+            cases = [("qft30", W.qft_u(30, seed=0)), ("brick30", W.brickwork_mixed(30, 24, seed=0)),
+                     ("random30", W.random_mixed(30, 40, seed=5))]
+            out["cases"] = CM.time_cases(mlp, cases, states, log=log)
         if "report" in stages:
             recs = []
             for name in sorted(os.listdir(ddir)):
@@ -519,13 +540,22 @@ def main():
                     help="qft30 (default), brick33, hea4096, brick20, qftNN")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage", default="all",
-                    help="planner workload: collect | train | eval | report | all")
+                    help="planner workload: collect | train | eval | report | cases | all")
     ap.add_argument("--shard", type=int, default=0, help="planner collect/eval: this shard")
     ap.add_argument("--shards", type=int, default=1, help="planner collect/eval: shard count")
     ap.add_argument("--data-dir", default="gpurun_out/planner",
                     help="planner: where collect/eval write and train/report read")
     ap.add_argument("--train-circuits", type=int, default=5000)
     ap.add_argument("--eval-circuits", type=int, default=200)
+    ap.add_argument("--ladder-circuits", type=int, default=0,
+                    help="planner collect: QFT-like circuits added to the training windows")
+    ap.add_argument("--ladder-window", type=int, default=1000,
+                    help="planner collect: gates in a QFT-like circuit's long window "
+                         "(the default takes the whole circuit)")
+    ap.add_argument("--init-model", default="",
+                    help="planner train: start from this model's parameters (fine-tuning)")
+    ap.add_argument("--epochs", type=int, default=60, help="planner train: epochs")
+    ap.add_argument("--lr", type=float, default=3e-3, help="planner train: peak learning rate")
     ap.add_argument("--model-out", default="cost_model_trained.npz",
                     help="planner train: where the trained cost model is written "
                          "(the shipped one is ket_amd/cost_model.npz)")

@@ -11,8 +11,15 @@ every kind of pass the planner's candidate plans hold. Timing whole
 28-32-qubit circuits would take hours; timing windows of them samples the
 same passes.
 
+Optionally (`bench.py --workload planner --ladder-circuits N`), QFT-like
+circuits contribute too, their long window (by default the whole circuit)
+timed as blocks of width 3, 4 and 5 as well as runs: blocks with dozens of
+trailing phase terms and runs of hundreds of phase passes, which the random
+circuits' windows do not hold.
+
 Model: the 16-64-64-1 MLP of include/ket_amd.h, trained on the GPU with Adam
-on log(milliseconds).
+on log(milliseconds), from a seeded start or from a previous model's
+parameters (fine-tuning on new timings).
 
 Evaluation: held-out circuits of the same distribution run whole in every
 form: the greedy block packer (KET_FUSE_BLOCKS, k = 5), ket's Fuser as exact
@@ -74,12 +81,16 @@ class States:
 SHORT_FORMS = ((N.FUSE_NONE, 5), (N.FUSE_EXACT, 5), (N.FUSE_BLOCKS_ALL, 3),
                (N.FUSE_BLOCKS_ALL, 4), (N.FUSE_BLOCKS_ALL, 5))
 LONG_FORMS = ((N.FUSE_RUNS, 5), (N.FUSE_REFERENCE, 5))
+# Phase ladders (QFT-like circuits): their blocks carry dozens of trailing
+# phase terms, which 8-gate windows of the random circuits never produce,
+# so their long windows are timed as blocks too.
+LADDER_FORMS = LONG_FORMS + ((N.FUSE_BLOCKS, 3), (N.FUSE_BLOCKS, 4), (N.FUSE_BLOCKS, 5))
 
 
 def collect(circuits, states: States, short_len: int = 8, long_len: int = 48, seed: int = 0,
-            log=None):
+            log=None, long_forms=LONG_FORMS):
     """Feature rows and log(ms) of every pass of both windows of each
-    circuit, in every form of SHORT_FORMS / LONG_FORMS."""
+    circuit, in every form of SHORT_FORMS / long_forms."""
     rng = np.random.default_rng(seed)
     X, y = [], []
     t0 = time.perf_counter()
@@ -87,7 +98,7 @@ def collect(circuits, states: States, short_len: int = 8, long_len: int = 48, se
         total = c.n_unitary()
         start = int(rng.integers(0, max(1, total - long_len)))
         sv = states.get(c.n_qubits)
-        for length, forms in ((short_len, SHORT_FORMS), (long_len, LONG_FORMS)):
+        for length, forms in ((short_len, SHORT_FORMS), (long_len, long_forms)):
             sub = window(c, start, length)
             for fusion, k in forms:
                 for r in sv.profile_circuit(sub, fusion, k):
@@ -100,8 +111,13 @@ def collect(circuits, states: States, short_len: int = 8, long_len: int = 48, se
     return np.array(X, np.float32).reshape(-1, N.MLP_INPUTS), np.array(y, np.float32)
 
 
-def train(X, y, epochs: int = 60, batch: int = 1024, lr: float = 3e-3, seed: int = 0, log=None):
+def train(X, y, epochs: int = 60, batch: int = 1024, lr: float = 3e-3, seed: int = 0, log=None,
+          init=None):
+    """Adam with a cosine learning rate; init: parameters to start from (a
+    model fine-tuned on new timings) instead of a seeded initialisation."""
     mlp = N.MLP(seed)
+    if init is not None:
+        mlp.params = init
     rng = np.random.default_rng(seed)
     n = len(X)
     hist = []
@@ -206,8 +222,40 @@ def summarize(recs: list) -> dict:
     }
 
 
+def time_cases(mlp: N.MLP, cases, states: States, reps: int = 3, log=None) -> dict:
+    """Named circuits in every form (median of `reps` cached runs each) and
+    the form the learned planner's plan takes (its launch and block counts)."""
+    out = {}
+    mlp.use_for_planning(True)
+    try:
+        for name, c in cases:
+            sv = states.get(c.n_qubits)
+            row = {}
+            for form, (fusion, k) in FORMS.items():
+                time_circuit(sv, c, fusion, k)  # plan and warm
+                row[form] = {"ms": float(np.median([time_circuit(sv, c, fusion, k)
+                                                    for _ in range(reps)])),
+                             "launches": sv.last_stats["launches"],
+                             "blocks": sv.last_stats["blocks"]}
+            out[name] = row
+            if log:
+                log(f"[costmodel] {name}: " + ", ".join(
+                    f"{f} {v['ms']:.1f} ms ({v['blocks']} blocks)" for f, v in row.items()))
+    finally:
+        mlp.use_for_planning(False)
+    return out
+
+
 def evaluate(mlp: N.MLP, circuits, states: States, reps: int = 3, log=None) -> dict:
     return summarize(time_forms(mlp, circuits, states, reps, log))
+
+
+def ladder_circuits(count: int, seed: int, lo: int = 28, hi: int = 32):
+    """QFT-like circuits: a random U layer, then the QFT's H + controlled-phase
+    ladders and closing SWAPs, on 28-32 qubits."""
+    rng = np.random.default_rng(seed)
+    return [W.qft_u(int(rng.integers(lo, hi + 1)), int(rng.integers(1 << 30)))
+            for _ in range(count)]
 
 
 def config5_circuits(count: int, seed: int, lo: int = 28, hi: int = 32, depth: int = 40):

@@ -74,3 +74,25 @@ def test_trained_planner_against_fixed_forms():
     tot, noise = res["total_ms"], res["noise"]
     assert tot["learned"] <= tot[res["best_fixed_form"]] * (1 + noise)
     assert tot["learned"] * (1 + noise) < tot[CM.FALLBACK]
+
+
+@pytest.mark.skipif(not os.path.exists(CM.MODEL_PATH), reason="no trained cost model committed")
+def test_trained_planner_takes_blocks_on_phase_ladders():
+    """Where the forms trade places: on a 30-qubit QFT (after a random U
+    layer, complex64) k = 5 blocks beat the run packer, on 30-qubit
+    brickwork the run packer beats blocks. The learned planner must be as
+    fast as the fastest fixed form on both (within 3%), so it beats the run
+    packer on the QFT and the block packer on the brickwork."""
+    mlp = CM.load()
+    states = CM.States(N.C64)
+    try:
+        res = CM.time_cases(mlp, [("qft", W.qft_u(30, seed=3)),
+                                  ("brick", W.brickwork_mixed(30, 24, seed=4))], states)
+    finally:
+        states.close()
+    print("learned planner on QFT and brickwork:", res)
+    for name, row in res.items():
+        best = min(v["ms"] for f, v in row.items() if f != "learned")
+        assert row["learned"]["ms"] <= best * 1.03, name
+    assert res["qft"]["learned"]["ms"] < 0.95 * res["qft"]["packed_runs"]["ms"]
+    assert res["brick"]["learned"]["ms"] < 0.95 * res["brick"]["greedy_blocks"]["ms"]
