@@ -122,6 +122,9 @@ __device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
                                                const unsigned char* __restrict__ rec,
                                                uint64_t block, uint64_t nblocks) {
   constexpr int D = 1 << K;
+  // two rows at a time for k <= 4: four independent FMA chains instead of
+  // two (the same VGPRs; c128 k = 4 passes 1% faster), one row at k >= 5
+  constexpr int kRowUnroll = K <= 4 ? 2 : 1;
   __shared__ ketd::cplx<T> M[D * D];
   __shared__ uint64_t off[D];
   __shared__ uint64_t tm[PH ? kMaxTerms : 1];
@@ -154,7 +157,7 @@ __device__ __forceinline__ void block_vec_body(ketd::cplx<T>* __restrict__ s,
       group_angles<K>(base, h, tm, th, C, A);
       ph.set(C, A);
     }
-#pragma unroll 1
+#pragma unroll kRowUnroll
     for (int r = 0; r < D; ++r) {
       T re = 0, im = 0;
 #pragma unroll
@@ -606,6 +609,7 @@ __device__ __forceinline__ void block_tile_body(ketd::cplx<T>* __restrict__ s, u
                                                 const unsigned char* __restrict__ rec,
                                                 uint64_t block, uint64_t nblocks) {
   constexpr int D = 1 << K;
+  constexpr int kRowUnroll = K <= 4 ? 2 : 1;  // as block_vec_body
   constexpr int NT = 1 << (TB - K);
   constexpr int SWB = sizeof(T) == 8 ? 3 : 4;
   using IO = TileIO<T, TB, NT>;
@@ -650,7 +654,7 @@ __device__ __forceinline__ void block_tile_body(ketd::cplx<T>* __restrict__ s, u
     for (int e = 0; e < D; ++e) in[e] = L[lg ^ qoff<K>(e, qbit)];
     // Each thread owns its group's LDS slots: it has read all of them
     // before it writes any, so no barrier is needed in between.
-#pragma unroll 1
+#pragma unroll kRowUnroll
     for (int r = 0; r < D; ++r) {
       T re = 0, im = 0;
 #pragma unroll
