@@ -1162,8 +1162,8 @@ __device__ __forceinline__ double uniform_f64(double x) {
 
 // a *= ph in place with ket's complex product: the four products, each
 // rounded, into temporaries, then x = xx - yy and y = xy + yx into a's own
-// registers. (Compiled, the real sum went to a fresh register and was
-// copied back: one more VALU instruction per amplitude and phase.)
+// registers (compiled, the phase block put the real sum in a fresh register
+// and copied it back).
 __device__ __forceinline__ void phase_mul(ketd::cplx<double>& a, ketd::cplx<double> ph) {
   double xx, yy, xy;
   __asm__("v_mul_f64 %2, %0, %5\n\t"
