@@ -312,8 +312,9 @@ typedef struct ket_tuning {
                                grid at 256*blocks_per_cu blocks (grid-stride) */
   int32_t block_tiles;      /* MFMA blocks: 16-group tiles each wave loads
                                before computing (1, 2; 0 = default 1) */
-  int32_t block_bpc;        /* fused blocks: workgroups per CU of the capped
-                               grid (MFMA: bpc, vector: 2*bpc; 0 = 32) */
+  int32_t block_bpc;        /* fused blocks and exact runs: workgroups per
+                               CU of the capped grid (MFMA and runs: bpc,
+                               vector blocks: 2*bpc; 0 = 32; default 64) */
   int32_t block_stage;      /* LDS tiles for fused blocks (16-byte accesses in
                                index order): complex64 blocks of width 4, 5
                                whenever this is > 0, other blocks of width

@@ -62,7 +62,7 @@ def probe_circuit(n: int) -> Circuit:
     return c
 
 
-BLOCK_SPACE = {"block_stage": [0, 3, 64], "block_bpc": [8, 16, 32]}
+BLOCK_SPACE = {"block_stage": [0, 3, 64], "block_bpc": [8, 16, 32, 64]}
 
 
 def block_probe(n: int, k: int) -> Circuit:

@@ -14,14 +14,18 @@ namespace {
 // line a pass writes is written whole (mode 3); the lane-pair exchange form
 // for partners fewer than 2^16 (complex128) or 2^4 (complex64) amplitudes
 // away.
-// Fused blocks: 32 workgroups per CU; complex64 blocks always through LDS
-// tiles, complex128 ones when their lowest qubit is 0..2 (always-tiles
-// measured within 0.1%), with nontemporal tile transfers. Exact runs keep
-// 3 (complex128) / 4 (complex64) low bits in every tile. Block matrices
+// Fused blocks and exact runs: grids capped at 64 workgroups per CU (the
+// autotuner's pick for blocks of both precisions; on QFT-30 also exact
+// runs 121.0 -> 118.0 ms complex128, 69.3 -> 67.6 ms complex64, against
+// 32); LDS tiles for blocks at any position (block_stage 64, the
+// autotuner's pick for both precisions; for complex128 within 0.5% of
+// block_stage 3, tiles only for blocks reaching into the lowest three
+// qubits), with nontemporal tile transfers. Exact runs keep 3 (complex128)
+// / 4 (complex64) low bits in every tile. Block matrices
 // are built on the GPU. Batched HBM instances: packed exact runs in
 // complex128 (4096-instance HEA sweep 0.578 s against 0.636 s as k = 4
 // blocks), fused k = 5 blocks in complex64 (0.332 s against 0.357 s).
-Tuning g_tuning[2] = {{1, 3, 0, 1, 32, 3, 16, 0, 1, 3, 1, 1}, {2, 3, 0, 1, 32, 64, 4, 0, 1, 4, 1, 0}};
+Tuning g_tuning[2] = {{1, 3, 0, 1, 64, 64, 16, 0, 1, 3, 1, 1}, {2, 3, 0, 1, 64, 64, 4, 0, 1, 4, 1, 0}};
 
 std::atomic<uint64_t> g_generation{1};
 
